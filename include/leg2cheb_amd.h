@@ -1,0 +1,164 @@
+/*
+ * leg2cheb_amd.h -- C ABI of the MI355X (gfx950) fast multipole
+ * Legendre<->Chebyshev transform.
+ *
+ * Drop-in for the plan API of spectralDNS/Legendre-to-Chebyshev
+ * (src/C/leg2cheb.h).  Every reference entry point keeps its name, argument
+ * list and argument meaning; the plan structs are opaque (the reference's
+ * own bindings already treat them as opaque pointers: src/cython/l2c.pyx:62,
+ * src/ctypes/l2ctypes.py:11-39 never read a field).
+ *
+ * Semantics that differ from the reference, by design:
+ *   - execute()/direct()/execute2D() OVERWRITE the output.  The reference's
+ *     FMM path accumulates into it (src/C/leg2cheb.c:1475,1480) and every
+ *     reference caller zeroes it first (src/cython/l2c.pyx:90,
+ *     src/ctypes/l2ctypes.py:108, src/bin/l2c.c:189), so zeroed callers see
+ *     identical results.
+ *   - errors never abort: a failing call returns NULL or (size_t)-1 and
+ *     l2c_last_error() describes the failure (the reference only asserts,
+ *     src/C/leg2cheb.c:1312).
+ *   - input and output may be host or device pointers (detected per call);
+ *     host pointers are staged through the plan's pinned buffers.
+ *   - a plan owns one HIP stream and its scratch; like the reference, one
+ *     plan must not be executed from two threads at once.
+ *
+ * Extensions (l2c_*) expose device-resident execution, batched and strided
+ * line execution, the split-precision form and plan introspection.
+ */
+#ifndef LEG2CHEB_AMD_H
+#define LEG2CHEB_AMD_H
+
+#include <stdbool.h>
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* src/C/leg2cheb.h:27 */
+enum { L2C = 0, C2L = 1, BOTH = 2 };
+
+typedef struct fmm_plan_s fmm_plan;       /* src/C/leg2cheb.h:58-78 (opaque) */
+typedef struct direct_plan_s direct_plan; /* src/C/leg2cheb.h:48-56 (opaque) */
+typedef struct fmm_plan_2d_s fmm_plan_2d; /* src/C/leg2cheb.h:80-86 (opaque) */
+
+/* ---- plan API: src/C/leg2cheb.h:88-105 ---------------------------------- */
+
+/* Replaces create_fmm, src/C/leg2cheb.c:820-1234 (decl. leg2cheb.h:94-96).
+ * Builds every table in HBM once: Lambda tables, the per-level Chebyshev
+ * (lagrange=0) or Lagrange (lagrange=1) M2L matrices, T, the M2M/L2L
+ * operators.  direction: L2C, C2L or BOTH.  precompute is accepted; the
+ * device band forms the same coefficient products on the fly.  v > 1 prints
+ * the plan summary like the reference. */
+fmm_plan *create_fmm(const size_t N, const size_t maxs, const size_t M,
+                     const size_t direction, const size_t lagrange,
+                     const size_t precompute, const size_t v);
+
+/* Replaces execute, src/C/leg2cheb.c:1299-1500 (decl. leg2cheb.h:104-105).
+ * Elements are input_array[i*stride], output_array[i*stride], i < N.
+ * Returns the reference's algorithmic flop count, (size_t)-1 on error. */
+size_t execute(const double *input_array, double *output_array,
+               fmm_plan *fmmplan, size_t direction, const size_t stride);
+
+/* Replaces free_fmm, src/C/leg2cheb.c:688-756 (decl. leg2cheb.h:88). */
+void free_fmm(fmm_plan *plan);
+
+/* Replaces create_direct / direct / free_direct, src/C/leg2cheb.c:758-818,
+ * :152-234, :650-673 (decl. leg2cheb.h:90, :98-99, :93).  The O(N^2) sums. */
+direct_plan *create_direct(size_t N, size_t direction, const size_t precompute);
+size_t direct(const double *input_array, double *output_array,
+              direct_plan *dplan, size_t direction, size_t stride);
+void free_direct(direct_plan *dplan);
+
+/* Replaces directM, src/C/leg2cheb.c:236-376 (decl. leg2cheb.h:100-101):
+ * the L2C near-field band alone, with the final 2/pi and b_0/2 scaling. */
+size_t directM(const double *input_array, double *output_array,
+               fmm_plan *fmmplan, const size_t strides);
+
+/* Replaces create_fmm_2d / execute2D / free_fmm_2d,
+ * src/C/leg2cheb.c:1236-1297, :675-686 (decl. leg2cheb.h:89, :91-93, :102-103).
+ * axis 0: along rows of an N0 x N1 row-major array (stride N1); axis 1: along
+ * each row (stride 1); axis -1: axis 0 then axis 1.  Runs every line of an
+ * axis in one batched launch. */
+fmm_plan_2d *create_fmm_2d(size_t N0, size_t N1, int axis, size_t maxs,
+                           size_t M, size_t direction, const size_t lagrange,
+                           const size_t precompute, size_t v);
+size_t execute2D(const double *input_array, double *output_array,
+                 fmm_plan_2d *fmmplan2d, size_t direction);
+void free_fmm_2d(fmm_plan_2d *plan);
+
+/* Replaces matvectri, src/C/leg2cheb.c:537-619 (host helper; compact
+ * triangular B(0) times the even/odd folded vector, as in the reference). */
+void matvectri(const double *A, const double *x, double *b, const size_t m,
+               const bool upper);
+
+/* Plan geometry helpers, src/C/leg2cheb.c:125-150 (decl. leg2cheb.h:108-114). */
+size_t get_number_of_blocks(const size_t level);
+size_t get_total_number_of_blocks(const size_t L);
+size_t get_h(const size_t level, const size_t L);
+size_t get_number_of_submatrices(const size_t level);
+size_t get_total_number_of_submatrices(const size_t L);
+void get_ij(size_t *ij, const size_t level, const size_t block, const size_t s,
+            const size_t L);
+
+/* Lambda(z) = Gamma(z+1/2)/Gamma(z+1): src/C/lambda.c:65-98, :101-192,
+ * :15-18, :20 (decl. leg2cheb.h:116-120).  Host functions; the device plan
+ * kernels evaluate the same expressions. */
+double _Lambda0(const double z);
+double Lambda(const double z);
+double LambdaI(const size_t z);
+void Lambdavec(const double *z, double *w, const size_t N);
+
+/* Replaces dct2, src/C/dct.c:582-598 (decl. leg2cheb.h:122): 18 x 18 2-D
+ * DCT-II scaled by 4/18^2 with row 0 and column 0 halved (host helper). */
+void dct2(double *input, double *output);
+
+/* ---- extensions (no counterpart in the reference) ----------------------- */
+
+/* Text of the last error on the calling thread ("" if none). */
+const char *l2c_last_error(void);
+
+/* Plan summary: N, Nn, L, s, M, number of submatrices, bytes of HBM held. */
+typedef struct {
+  size_t N, Nn, L, s, M, maxs, direction, lagrange, Nc, Nb;
+  size_t hbm_bytes;    /* device memory held by the plan */
+  size_t a_bytes;      /* bytes of one direction's M2L matrices */
+  int subtree_up;      /* leaf chunks per upward work-group */
+  int subtree_down;    /* leaf chunks per downward work-group */
+} l2c_plan_info;
+int l2c_get_plan_info(const fmm_plan *plan, l2c_plan_info *info);
+
+/* Device-resident execute: d_in, d_out are device pointers; runs on `stream`
+ * (a hipStream_t, NULL = the plan's own stream) and returns without
+ * synchronising.  Same overwrite semantics.  Returns 0 or -1. */
+int l2c_execute_device(fmm_plan *plan, const double *d_in, double *d_out,
+                       size_t direction, size_t stride, void *stream);
+
+/* Many lines of length N through one plan in one launch.  Line l (l < nlines)
+ * has first element at (l / n0) * line_stride1 + (l % n0) * line_stride0 and
+ * its elements at that offset + i * elem_stride.  Device pointers.  With
+ * n0 = nlines, line_stride0 = distance between consecutive lines. */
+int l2c_execute_lines_device(fmm_plan *plan, const double *d_in, double *d_out,
+                             size_t direction, size_t nlines, size_t n0,
+                             size_t line_stride0, size_t line_stride1,
+                             size_t elem_stride, void *stream);
+
+/* Precision form of a plan: 0 = plain FP64 (default); 1 = split precision
+ * (M2L and leaf products as BF16-sliced MFMA products, FP32 accumulation,
+ * FP64 recombination; Lambda tables, near-field band and DCT stay FP64). */
+int l2c_set_precision(fmm_plan *plan, int form);
+
+/* Synchronise the plan's stream. */
+int l2c_synchronize(fmm_plan *plan);
+
+/* Device memory helpers for hosts without their own allocator. */
+void *l2c_malloc_device(size_t bytes);
+int l2c_free_device(void *p);
+int l2c_memcpy_h2d(void *dst, const void *src, size_t bytes);
+int l2c_memcpy_d2h(void *dst, const void *src, size_t bytes);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,355 @@
+"""Python host interface of the gfx950 FMM Legendre<->Chebyshev library.
+
+Mirrors the reference's Python bindings (src/cython/l2c.pyx:17-173 and
+src/ctypes/l2ctypes.py:53-113): a ``Leg2Cheb`` plan class called with
+``direction`` 0 (L2C) or 1 (C2L), the O(N^2) ``leg2cheb``/``cheb2leg`` and
+``Lambdav``.  Everything runs through the C ABI of libleg2cheb_amd.so
+(include/leg2cheb_amd.h); there is no CPU fallback -- the loader raises if the
+library is missing.
+
+The directory name carries hyphens, so load it under a module name of your
+choice, e.g.::
+
+    spec = importlib.util.spec_from_file_location(
+        "l2c_amd", ".../legendre-to-chebyshev_amd/__init__.py",
+        submodule_search_locations=[".../legendre-to-chebyshev_amd"])
+"""
+import ctypes as C
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libleg2cheb_amd.so")
+
+L2C, C2L, BOTH = 0, 1, 2
+_dp = C.POINTER(C.c_double)
+_sz = C.c_size_t
+_SIZE_MAX = (1 << 64) - 1
+
+
+class L2CError(RuntimeError):
+    pass
+
+
+class PlanInfo(C.Structure):
+    _fields_ = [(n, _sz) for n in ("N", "Nn", "L", "s", "M", "maxs", "direction", "lagrange",
+                                   "Nc", "Nb", "hbm_bytes", "a_bytes")] + [
+        ("subtree_up", C.c_int), ("subtree_down", C.c_int)]
+
+
+_lib = None
+
+
+def lib():
+    """The loaded C library (raises L2CError when it has not been built)."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise L2CError(f"{LIB_PATH} is missing: build it with `make -C legendre-to-chebyshev_amd`")
+        L = C.CDLL(LIB_PATH)
+        vp = C.c_void_p
+        L.create_fmm.restype = vp
+        L.create_fmm.argtypes = [_sz] * 7
+        L.execute.restype = _sz
+        L.execute.argtypes = [vp, vp, vp, _sz, _sz]
+        L.free_fmm.argtypes = [vp]
+        L.create_direct.restype = vp
+        L.create_direct.argtypes = [_sz, _sz, _sz]
+        L.direct.restype = _sz
+        L.direct.argtypes = [vp, vp, vp, _sz, _sz]
+        L.free_direct.argtypes = [vp]
+        L.directM.restype = _sz
+        L.directM.argtypes = [vp, vp, vp, _sz]
+        L.create_fmm_2d.restype = vp
+        L.create_fmm_2d.argtypes = [_sz, _sz, C.c_int, _sz, _sz, _sz, _sz, _sz, _sz]
+        L.execute2D.restype = _sz
+        L.execute2D.argtypes = [vp, vp, vp, _sz]
+        L.free_fmm_2d.argtypes = [vp]
+        L.Lambda.restype = C.c_double
+        L.Lambda.argtypes = [C.c_double]
+        L._Lambda0.restype = C.c_double
+        L._Lambda0.argtypes = [C.c_double]
+        L.LambdaI.restype = C.c_double
+        L.LambdaI.argtypes = [_sz]
+        L.Lambdavec.argtypes = [vp, vp, _sz]
+        L.matvectri.argtypes = [vp, vp, vp, _sz, C.c_bool]
+        L.dct2.argtypes = [vp, vp]
+        for f in ("get_number_of_blocks", "get_total_number_of_blocks",
+                  "get_number_of_submatrices", "get_total_number_of_submatrices"):
+            getattr(L, f).restype = _sz
+            getattr(L, f).argtypes = [_sz]
+        L.get_h.restype = _sz
+        L.get_h.argtypes = [_sz, _sz]
+        L.get_ij.argtypes = [C.POINTER(_sz), _sz, _sz, _sz, _sz]
+        L.l2c_last_error.restype = C.c_char_p
+        L.l2c_get_plan_info.restype = C.c_int
+        L.l2c_get_plan_info.argtypes = [vp, C.POINTER(PlanInfo)]
+        L.l2c_execute_device.restype = C.c_int
+        L.l2c_execute_device.argtypes = [vp, vp, vp, _sz, _sz, vp]
+        L.l2c_execute_lines_device.restype = C.c_int
+        L.l2c_execute_lines_device.argtypes = [vp, vp, vp, _sz, _sz, _sz, _sz, _sz, _sz, vp]
+        L.l2c_set_precision.restype = C.c_int
+        L.l2c_set_precision.argtypes = [vp, C.c_int]
+        L.l2c_synchronize.restype = C.c_int
+        L.l2c_synchronize.argtypes = [vp]
+        L.l2c_malloc_device.restype = vp
+        L.l2c_malloc_device.argtypes = [_sz]
+        L.l2c_free_device.restype = C.c_int
+        L.l2c_free_device.argtypes = [vp]
+        L.l2c_memcpy_h2d.restype = C.c_int
+        L.l2c_memcpy_h2d.argtypes = [vp, vp, _sz]
+        L.l2c_memcpy_d2h.restype = C.c_int
+        L.l2c_memcpy_d2h.argtypes = [vp, vp, _sz]
+        _lib = L
+    return _lib
+
+
+def last_error():
+    return lib().l2c_last_error().decode()
+
+
+def _check(ret, what):
+    if ret is None or ret == _SIZE_MAX or (isinstance(ret, int) and ret < 0 and ret != _SIZE_MAX and what.endswith("rc")):
+        raise L2CError(f"{what}: {last_error()}")
+    return ret
+
+
+def _ptr(a):
+    if not (a.dtype == np.float64 and a.flags.c_contiguous):
+        raise ValueError("arrays must be C-contiguous float64")
+    return a.ctypes.data
+
+
+class Leg2Cheb:
+    """Plan for Legendre<->Chebyshev transforms (src/cython/l2c.pyx:17-114).
+
+    Parameters mirror the reference: input_array/output_array (float64, 1-D),
+    M (rank), maxs (max leaf size), direction (0 L2C, 1 C2L, 2 both),
+    lagrange, precompute, verbose.  Calls overwrite the output array.
+    """
+
+    def __init__(self, input_array, output_array, M=18, maxs=36, direction=BOTH, lagrange=0,
+                 precompute=0, verbose=1):
+        self.N = int(input_array.shape[0])
+        self.direction = direction
+        self._plan = lib().create_fmm(self.N, maxs, M, direction, lagrange, precompute, verbose)
+        if not self._plan:
+            raise L2CError("create_fmm: " + last_error())
+        self._input_array = input_array
+        self._output_array = output_array
+        self.flops = 0
+
+    def __call__(self, input_array=None, output_array=None, direction=0):
+        assert direction in (0, 1), "Please specify direction of transform!"
+        if input_array is not None:
+            self._input_array[...] = input_array
+        f = lib().execute(_ptr(self._input_array), _ptr(self._output_array), self._plan,
+                          direction, 1)
+        if f == _SIZE_MAX:
+            raise L2CError("execute: " + last_error())
+        self.flops = f
+        if output_array is not None:
+            output_array[...] = self._output_array
+            return output_array
+        return self._output_array
+
+    @property
+    def input_array(self):
+        return self._input_array
+
+    @property
+    def output_array(self):
+        return self._output_array
+
+    @property
+    def plan(self):
+        return self._plan
+
+    def info(self):
+        inf = PlanInfo()
+        if lib().l2c_get_plan_info(self._plan, C.byref(inf)) != 0:
+            raise L2CError(last_error())
+        return {k: getattr(inf, k) for k, _ in PlanInfo._fields_}
+
+    def __del__(self):
+        if getattr(self, "_plan", None):
+            lib().free_fmm(self._plan)
+            self._plan = None
+
+
+class FMMPlan:
+    """Thin wrapper over create_fmm/execute/free_fmm with explicit stride."""
+
+    def __init__(self, N, maxs=64, M=18, direction=BOTH, lagrange=0, precompute=0, verbose=0):
+        self.N = N
+        self._plan = lib().create_fmm(N, maxs, M, direction, lagrange, precompute, verbose)
+        if not self._plan:
+            raise L2CError("create_fmm: " + last_error())
+
+    def execute(self, u, direction, stride=1, out=None):
+        u = np.ascontiguousarray(u, dtype=np.float64)
+        if out is None:
+            out = np.zeros_like(u)
+        f = lib().execute(_ptr(u), _ptr(out), self._plan, direction, stride)
+        if f == _SIZE_MAX:
+            raise L2CError("execute: " + last_error())
+        self.flops = f
+        return out
+
+    def directM(self, u, stride=1):
+        u = np.ascontiguousarray(u, dtype=np.float64)
+        out = np.zeros_like(u)
+        f = lib().directM(_ptr(u), _ptr(out), self._plan, stride)
+        if f == _SIZE_MAX:
+            raise L2CError("directM: " + last_error())
+        return out
+
+    def execute_device(self, d_in, d_out, direction, stride=1, stream=None):
+        rc = lib().l2c_execute_device(self._plan, d_in, d_out, direction, stride, stream)
+        if rc != 0:
+            raise L2CError("l2c_execute_device: " + last_error())
+
+    def execute_lines_device(self, d_in, d_out, direction, nlines, n0, ls0, ls1, es, stream=None):
+        rc = lib().l2c_execute_lines_device(self._plan, d_in, d_out, direction, nlines, n0, ls0,
+                                            ls1, es, stream)
+        if rc != 0:
+            raise L2CError("l2c_execute_lines_device: " + last_error())
+
+    def set_precision(self, form):
+        if lib().l2c_set_precision(self._plan, form) != 0:
+            raise L2CError(last_error())
+
+    def synchronize(self):
+        if lib().l2c_synchronize(self._plan) != 0:
+            raise L2CError(last_error())
+
+    def info(self):
+        inf = PlanInfo()
+        if lib().l2c_get_plan_info(self._plan, C.byref(inf)) != 0:
+            raise L2CError(last_error())
+        return {k: getattr(inf, k) for k, _ in PlanInfo._fields_}
+
+    @property
+    def plan(self):
+        return self._plan
+
+    def __del__(self):
+        if getattr(self, "_plan", None):
+            lib().free_fmm(self._plan)
+            self._plan = None
+
+
+class FMMPlan2D:
+    """create_fmm_2d / execute2D / free_fmm_2d (src/C/leg2cheb.c:1236-1297)."""
+
+    def __init__(self, N0, N1, axis, maxs=64, M=18, direction=BOTH, lagrange=0, precompute=0):
+        self._plan = lib().create_fmm_2d(N0, N1, axis, maxs, M, direction, lagrange, precompute, 0)
+        if not self._plan:
+            raise L2CError("create_fmm_2d: " + last_error())
+
+    def execute(self, u, direction, out=None):
+        u = np.ascontiguousarray(u, dtype=np.float64)
+        if out is None:
+            out = np.zeros_like(u)
+        f = lib().execute2D(_ptr(u), _ptr(out), self._plan, direction)
+        if f == _SIZE_MAX:
+            raise L2CError("execute2D: " + last_error())
+        return out
+
+    def __del__(self):
+        if getattr(self, "_plan", None):
+            lib().free_fmm_2d(self._plan)
+            self._plan = None
+
+
+def _direct(input_array, output_array, direction):
+    n = int(input_array.shape[0])
+    p = lib().create_direct(n, direction, 0)
+    if not p:
+        raise L2CError("create_direct: " + last_error())
+    try:
+        f = lib().direct(_ptr(input_array), _ptr(output_array), p, direction, 1)
+        if f == _SIZE_MAX:
+            raise L2CError("direct: " + last_error())
+    finally:
+        lib().free_direct(p)
+    return output_array
+
+
+def leg2cheb(input_array, output_array):
+    """O(N^2) Legendre to Chebyshev (src/cython/l2c.pyx:116-135)."""
+    return _direct(input_array, output_array, L2C)
+
+
+def cheb2leg(input_array, output_array):
+    """O(N^2) Chebyshev to Legendre (src/cython/l2c.pyx:137-154)."""
+    return _direct(input_array, output_array, C2L)
+
+
+def Lambdav(x):
+    """Lambda(x) = Gamma(x+1/2)/Gamma(x+1) elementwise (src/cython/l2c.pyx:156-173)."""
+    x = np.ascontiguousarray(x, dtype=np.float64)
+    a = np.empty(x.size)
+    lib().Lambdavec(x.ctypes.data, a.ctypes.data, x.size)
+    return a.reshape(np.shape(x))
+
+
+def Lambda(z):
+    return lib().Lambda(float(z))
+
+
+def LambdaI(k):
+    return lib().LambdaI(int(k))
+
+
+class DeviceBuffer:
+    """A device allocation made by the library (for device-resident runs)."""
+
+    def __init__(self, n):
+        self.n = n
+        self.ptr = lib().l2c_malloc_device(n * 8)
+        if not self.ptr:
+            raise L2CError("l2c_malloc_device: " + last_error())
+
+    def upload(self, a):
+        a = np.ascontiguousarray(a, dtype=np.float64)
+        assert a.size <= self.n
+        if lib().l2c_memcpy_h2d(self.ptr, a.ctypes.data, a.size * 8) != 0:
+            raise L2CError(last_error())
+
+    def download(self, n=None):
+        n = self.n if n is None else n
+        out = np.empty(n)
+        if lib().l2c_memcpy_d2h(out.ctypes.data, self.ptr, n * 8) != 0:
+            raise L2CError(last_error())
+        return out
+
+    def free(self):
+        if self.ptr:
+            lib().l2c_free_device(self.ptr)
+            self.ptr = None
+
+    def __del__(self):
+        self.free()
+
+
+def splitmix_uniform(n, seed, lo=-1.0, hi=1.0):
+    """n seeded uniforms in [lo, hi) from the SplitMix64 counter generator:
+    z_i = mix(seed + (i+1) * 0x9E3779B97F4A7C15), U = (z >> 11) * 2^-53."""
+    i = np.arange(1, n + 1, dtype=np.uint64)
+    with np.errstate(over="ignore"):
+        z = np.uint64(seed) + i * np.uint64(0x9E3779B97F4A7C15)
+        z = (z ^ (z >> np.uint64(30))) * np.uint64(0xBF58476D1CE4E5B9)
+        z = (z ^ (z >> np.uint64(27))) * np.uint64(0x94D049BB133111EB)
+        z = z ^ (z >> np.uint64(31))
+    u = (z >> np.uint64(11)).astype(np.float64) * (1.0 / 9007199254740992.0)
+    return lo + (hi - lo) * u
+
+
+def coefficients(n, seed, decay=0.0):
+    """Seeded Legendre coefficients U(-1,1) * (k+1)^-decay, k < n."""
+    u = splitmix_uniform(n, seed)
+    if decay:
+        u = u / np.power(np.arange(1, n + 1, dtype=np.float64), decay)
+    return u

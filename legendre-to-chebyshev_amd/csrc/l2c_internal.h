@@ -1,0 +1,154 @@
+// l2c_internal.h -- plan layout, geometry and the shared host/device Lambda
+// of the gfx950 FMM Legendre<->Chebyshev library.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstddef>
+#include <cstdint>
+#include <string>
+
+#include "../../include/leg2cheb_amd.h"
+
+namespace l2c {
+
+// ---------------------------------------------------------------------------
+// errors: thread-local text, returned by l2c_last_error()
+void set_error(const std::string &msg);
+void clear_error();
+
+#define L2C_HIP(call, ret)                                                   \
+  do {                                                                       \
+    hipError_t e_ = (call);                                                  \
+    if (e_ != hipSuccess) {                                                  \
+      ::l2c::set_error(std::string(#call) + ": " + hipGetErrorString(e_));   \
+      return ret;                                                            \
+    }                                                                        \
+  } while (0)
+
+// ---------------------------------------------------------------------------
+// Lambda(z) = Gamma(z+1/2)/Gamma(z+1), same expressions on host and device
+// (src/C/lambda.c:65-98).  Contraction is off so host and device round alike.
+__host__ __device__ inline double lambda_series(double z) {
+#pragma clang fp contract(off)
+  const double zp = z + 0.25;
+  const double sc = 1 / sqrt(zp);
+  const double w = 1 / (zp * zp);
+  // coefficients of the expansion in (z+1/4)^-2 (FMM_paper.pdf eq. 4.8)
+  double y = 1 + (-1.0 / 64.0) * w;
+  if (z > 2014) return y * sc;
+  double p = w * w;
+  y += (21.0 / 8192.0) * p;
+  if (z > 141) return y * sc;
+  p *= w;
+  y += (-671.0 / 524288.0) * p;
+  if (z > 40) return y * sc;
+  p *= w;
+  y += (180323.0 / 134217728.0) * p;
+  if (z > 20) return y * sc;
+  p *= w;
+  y += (-20898423.0 / 8589934592.0) * p;
+  if (z > 14) return y * sc;
+  p *= w;
+  y += (7426362705.0 / 1099511627776.0) * p;
+  return y * sc;
+}
+
+// Below 10 the argument is raised by repeated +1 and the recursion
+// Lambda(z) = Lambda(z+1)(z+1)/(z+1/2) unwound innermost first.
+__host__ __device__ inline double lambda(double z) {
+#pragma clang fp contract(off)
+  int n = 0;
+  double zz = z;
+  while (zz < 10 && n < 64) {
+    zz = zz + 1;
+    n++;
+  }
+  double v = lambda_series(zz);
+  // regenerate the same argument sequence from the bottom up
+  for (int k = n - 1; k >= 0; k--) {
+    double zk = z;
+    for (int t = 0; t < k; t++) zk = zk + 1;
+    v = v * (zk + 1) / (zk + 0.5);
+  }
+  return v;
+}
+
+// ---------------------------------------------------------------------------
+// Plan geometry (src/C/leg2cheb.c:125-150, :860-871).
+__host__ __device__ inline size_t nblocks(size_t level) { return ((size_t)2 << level) - 1; }
+__host__ __device__ inline size_t total_blocks(size_t L) { return ((size_t)2 << L) - (L + 2); }
+__host__ __device__ inline size_t total_sub(size_t L) { return 3 * total_blocks(L); }
+// Chunk slots of the multipole store: level l holds 2^(l+2) chunks.
+__host__ __device__ inline size_t slot_base(int level) { return ((size_t)4 << level) - 4; }
+
+struct Geo {
+  int N, Nn, L, s, M;
+  int nL;      // N / (2s): number of 2s-row near-field blocks
+  int gu, lu;  // upward work-group: 2^gu leaf chunks; cut level lu = L-1-gu
+  int gd, ld;  // downward work-group: 2^gd leaf chunks; cut level ld = L-1-gd
+};
+
+struct DevTables {
+  double *a = nullptr;   // Lambda(k), k < N
+  double *an = nullptr;  // C2L diagonal scale
+  double *dn = nullptr;  // C2L band factor
+  double *A[2] = {nullptr, nullptr};  // Nc x M x M per direction
+  double *T = nullptr;   // 2 x s x M  [parity][row][k]
+  double *U = nullptr;   // 2 x M x M  upward (child q -> parent), row-major
+  double *D = nullptr;   // 2 x M x M  downward (parent -> child p), row-major
+  double *P = nullptr;   // composite upward operators for the top levels
+  double *Asl[2] = {nullptr, nullptr};  // split-precision M2L slices
+};
+
+}  // namespace l2c
+
+struct direct_plan_s {
+  size_t N = 0, direction = 0, precompute = 0;
+  int device = 0;
+  double *d_a = nullptr, *d_an = nullptr, *d_dn = nullptr;
+  double *d_in = nullptr, *d_out = nullptr;  // staging (device)
+  double *h_in = nullptr, *h_out = nullptr;  // staging (pinned host)
+  hipStream_t stream = nullptr;
+};
+
+struct fmm_plan_s {
+  size_t N = 0, maxs = 0, M = 0, direction = 0, lagrange = 0, precompute = 0;
+  size_t Nn = 0, L = 0, s = 0, Nc = 0, Nb = 0;
+  int device = 0;
+  int precision = 0;  // 0 plain FP64, 1 split precision
+  l2c::Geo geo{};
+  l2c::DevTables t;
+  direct_plan_s *dplan = nullptr;  // L < 1: direct path only
+  double *d_w = nullptr;           // multipole store: slots x 2 x M
+  double *d_wl = nullptr;          // multipole store for line batches
+  size_t wl_cap = 0;               // doubles held by d_wl
+  double *d_in = nullptr, *d_out = nullptr;  // staging (device, contiguous)
+  double *h_in = nullptr, *h_out = nullptr;  // staging (pinned host)
+  size_t hbm_bytes = 0;
+  hipStream_t stream = nullptr;
+};
+
+struct fmm_plan_2d_s {
+  fmm_plan_s *p0 = nullptr, *p1 = nullptr;
+  size_t N0 = 0, N1 = 0;
+  int axis = 0;
+  double *d_in = nullptr, *d_tmp = nullptr, *d_out = nullptr;
+};
+
+namespace l2c {
+// implemented in l2c_exec.hip
+int run_fmm(fmm_plan_s *p, const double *d_in, double *d_out, size_t dir, size_t stride,
+            hipStream_t st, int band_only);
+int run_direct(const direct_plan_s *dp, const double *d_in, double *d_out, size_t dir,
+               size_t stride, hipStream_t st, size_t precompute);
+int run_lines(fmm_plan_s *p, const double *d_in, double *d_out, size_t dir, size_t nlines,
+              size_t n0, size_t ls0, size_t ls1, size_t es, hipStream_t st);
+size_t flops_fmm(const fmm_plan_s *p, size_t dir);
+size_t down_shared_bytes(const Geo &g);
+size_t up_shared_bytes(const Geo &g);
+size_t flops_direct(size_t N, size_t dir, size_t precompute);
+// implemented in l2c_plan.hip
+direct_plan_s *make_direct(size_t N, size_t direction, size_t precompute, bool staging);
+void destroy_direct(direct_plan_s *dp);
+}  // namespace l2c

@@ -1,0 +1,124 @@
+"""GPU parity: the HIP path against the oracle (CPU restatement of the
+reference's execute, src/C/leg2cheb.c:1299-1500) and the extended-precision
+direct sums.  Tolerance: relative max-norm error <= 1e-14 * log2(N)."""
+import math
+
+import numpy as np
+import pytest
+
+from conftest import announce, relerr
+
+pytestmark = pytest.mark.gpu
+
+
+def tol(N):
+    return 1e-14 * math.log2(max(N, 2))
+
+
+CASES = [(257, 64), (300, 32), (512, 64), (1000, 32), (1025, 64), (4096, 64), (8191, 64),
+         (8192, 32), (12345, 64), (65536, 64)]
+
+
+@pytest.mark.parametrize("N,maxs", CASES)
+@pytest.mark.parametrize("direction", [0, 1])
+def test_fmm_matches_oracle(amd, orc, N, maxs, direction):
+    announce(f"fmm N={N} maxs={maxs} dir={direction}")
+    u = amd.coefficients(N, seed=1000 + N, decay=0.0 if N % 2 else 1.5)
+    g = amd.FMMPlan(N, maxs, 18, direction).execute(u, direction)
+    o = orc.FMMPlan(N, maxs, 18, direction).execute(u, direction)
+    e = relerr(g, o)
+    print(f"  rel err vs oracle FMM = {e:.3e}")
+    assert e <= tol(N)
+    if N <= 8192:
+        ref = orc.leg2cheb_dd(u) if direction == 0 else orc.cheb2leg_dd(u)
+        e2 = relerr(g, ref)
+        print(f"  rel err vs double-double direct = {e2:.3e}")
+        assert e2 <= tol(N)
+
+
+@pytest.mark.parametrize("N", [1, 2, 3, 20, 64, 100, 255, 256])
+@pytest.mark.parametrize("direction", [0, 1])
+def test_direct_path(amd, orc, N, direction):
+    announce(f"direct-path N={N} dir={direction}")
+    u = amd.coefficients(N, seed=7 + N)
+    g = amd.FMMPlan(N, 64, 18, direction).execute(u, direction)
+    o = orc.FMMPlan(N, 64, 18, direction).execute(u, direction)
+    assert relerr(g, o) <= tol(N)
+    d = amd.leg2cheb(u, np.zeros(N)) if direction == 0 else amd.cheb2leg(u, np.zeros(N))
+    assert relerr(d, o) <= tol(N)
+
+
+def test_known_answer_N20(amd):
+    """src/bin/l2c.c:69-74, 94, 104."""
+    announce("KAT N=20")
+    a = [1.7913439415860921, 2.3408718378050253, 1.8833606631960720, 1.6587642683880404,
+         1.4417173322290182]
+    b = [0.52631578947368418, 0.075187969924812026, 0.13268465280849182, 0.17768205680441512,
+         0.24367824933176932]
+    u = np.ones(20)
+    p = amd.FMMPlan(20, 64, 18, 2)
+    assert np.linalg.norm(p.execute(u, 0)[:5] - a) < 1e-7
+    assert np.linalg.norm(p.execute(u, 1)[:5] - b) < 1e-7
+
+
+@pytest.mark.parametrize("N,maxs", [(1000, 32), (512, 64), (100, 32), (10000, 36)])
+def test_round_trip(amd, N, maxs):
+    """src/bin/l2c.c:10-65 (-d2): max error < 1e-10."""
+    announce(f"round trip N={N} maxs={maxs}")
+    u = amd.coefficients(N, seed=3) / np.arange(1, N + 1)
+    p = amd.FMMPlan(N, maxs, 18, 2)
+    b = p.execute(u, 0)
+    c = p.execute(b, 1)
+    assert np.max(np.abs(c - u)) < 1e-10
+
+
+@pytest.mark.parametrize("N,stride", [(1000, 3), (4097, 2), (200, 5)])
+def test_strided(amd, orc, N, stride):
+    announce(f"strided N={N} stride={stride}")
+    u = amd.coefficients(N, seed=11)
+    big = np.zeros(N * stride)
+    big[::stride] = u
+    p = amd.FMMPlan(N, 64, 18, 2)
+    for d in (0, 1):
+        out = np.full(N * stride, 7.0)
+        p.execute(big, d, stride=stride, out=out)
+        ref = orc.FMMPlan(N, 64, 18, 2).execute(u, d)
+        assert relerr(out[::stride], ref) <= tol(N)
+        mask = np.ones(N * stride, bool)
+        mask[::stride] = False
+        assert np.all(out[mask] == 7.0)
+
+
+@pytest.mark.parametrize("lagrange,precompute", [(1, 0), (0, 1), (1, 1)])
+def test_variants(amd, orc, lagrange, precompute):
+    N = 3000
+    announce(f"variant lagrange={lagrange} precompute={precompute}")
+    u = amd.coefficients(N, seed=5)
+    for d in (0, 1):
+        g = amd.FMMPlan(N, 32, 18, d, lagrange, precompute).execute(u, d)
+        o = orc.FMMPlan(N, 32, 18, d, lagrange, precompute).execute(u, d)
+        assert relerr(g, o) <= tol(N)
+
+
+def test_directM(amd, orc):
+    N = 5000
+    announce("directM")
+    u = amd.coefficients(N, seed=9)
+    g = amd.FMMPlan(N, 64, 18, 0).directM(u)
+    o = np.zeros(N)
+    op = orc.FMMPlan(N, 64, 18, 0)
+    orc.lib().orc_directM(orc._p(u), orc._p(o), op._p, 1)
+    assert relerr(g, o) <= tol(N)
+
+
+@pytest.mark.parametrize("axis", [0, 1, -1])
+def test_2d(amd, orc, axis):
+    """src/bin/l2c.c:297-393 (-d4, -d5)."""
+    N0, N1 = 400, 402
+    announce(f"2d axis={axis}")
+    rng = np.random.default_rng(1)
+    x = rng.uniform(-1, 1, (N0, N1))
+    for d in (0, 1):
+        g = amd.FMMPlan2D(N0, N1, axis, 64).execute(x, d)
+        o = orc.FMMPlan2D(N0, N1, axis, 64).execute(x, d)
+        assert relerr(g, o) <= tol(max(N0, N1))
