@@ -1,0 +1,182 @@
+#!/usr/bin/env python3
+"""L2C transforms/s at N = 2^20 on one MI355X (plain FP64 FMM, maxs = 64, M = 18).
+
+A step is one Legendre->Chebyshev transform of one synthetic coefficient
+vector (U(-1,1), SplitMix64 seed 1) that is already resident in HBM.  Warmup
+steps are untimed; the timed region is bracketed by device synchronisation.
+Every execute kernel is timed live with HIP events on its launch stream; the
+dominant kernel's average duration gives the roofline figure.  The result is
+checked against the oracle (CPU restatement of the reference's execute) and
+against extended-precision direct sums on sampled rows.  The oracle, run on
+the host cores, is the CPU baseline.
+
+    python bench.py [--steps K] [--warmup W] [--gpus 1]
+"""
+import argparse
+import importlib.util
+import json
+import math
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+PKG_DIR = os.path.join(ROOT, "legendre-to-chebyshev_amd")
+HBM_PEAK_GBS = 8000.0
+METRIC = "L2C transforms/s at N=2^20 and batched N<=8192; max rel err vs FP64 oracle"
+
+
+def load_pkg():
+    if "l2c_amd" in sys.modules:
+        return sys.modules["l2c_amd"]
+    spec = importlib.util.spec_from_file_location(
+        "l2c_amd", os.path.join(PKG_DIR, "__init__.py"), submodule_search_locations=[PKG_DIR])
+    mod = importlib.util.module_from_spec(spec)
+    sys.modules["l2c_amd"] = mod
+    spec.loader.exec_module(mod)
+    return mod
+
+
+def log(msg):
+    print(msg, file=sys.stderr, flush=True)
+
+
+def down_band_bytes(info, N, direction):
+    """Algorithmic HBM bytes of one k_down_band launch: the M2L matrices
+    (Nc*M^2), input, output, the Lambda table(s) the band reads (a for L2C;
+    an for C2L, dn is a 2s-entry head) and the multipole store (slots*2*M)."""
+    M, L = info["M"], info["L"]
+    slots = (4 << L) - 4
+    return 8 * (info["Nc"] * M * M + 3 * N + slots * 2 * M)
+
+
+def pmc_traffic(workload):
+    """HBM bytes per k_down_band launch from the committed rocprofv3 --pmc
+    summary of this workload, if present (profiles/<kernel>_<workload>_pmc.json)."""
+    path = os.path.join(ROOT, "profiles", f"k_down_band_{workload}_pmc.json")
+    if not os.path.exists(path):
+        return None
+    with open(path) as f:
+        return json.load(f).get("hbm_bytes_per_launch")
+
+
+def main():
+    ap = argparse.ArgumentParser(description=__doc__, formatter_class=argparse.RawDescriptionHelpFormatter)
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=400)
+    ap.add_argument("--warmup", type=int, default=40)
+    ap.add_argument("--N", type=int, default=1 << 20)
+    ap.add_argument("--maxs", type=int, default=64)
+    ap.add_argument("--M", type=int, default=18)
+    ap.add_argument("--direction", type=int, default=0, choices=(0, 1))
+    ap.add_argument("--cpu-seconds", type=float, default=10.0,
+                    help="bounded CPU-baseline sample (seconds of oracle work)")
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+    ap.add_argument("--check-rows", type=int, default=512,
+                    help="rows checked against the double-double direct sum")
+    args = ap.parse_args()
+
+    world = int(os.environ.get("WORLD_SIZE", "1"))
+    if args.gpus != 1 or world != 1:
+        print("bench.py: this build runs on one GPU only; --gpus must be 1 "
+              "(sharding across GPUs is described in DESIGN.md, not launched).", file=sys.stderr)
+        sys.exit(2)
+
+    amd = load_pkg()
+    N, maxs, M, d = args.N, args.maxs, args.M, args.direction
+    u = amd.coefficients(N, seed=1)
+
+    log(f"[bench] plan N={N} maxs={maxs} M={M} direction={d}")
+    t0 = time.perf_counter()
+    plan = amd.FMMPlan(N, maxs, M, d)
+    plan_s = time.perf_counter() - t0
+    info = plan.info()
+    din, dout = amd.DeviceBuffer(N), amd.DeviceBuffer(N)
+    din.upload(u)
+
+    log(f"[bench] warmup {args.warmup}")
+    for _ in range(args.warmup):
+        plan.execute_device(din.ptr, dout.ptr, d)
+    plan.synchronize()
+
+    log(f"[bench] timed {args.steps} steps")
+    plan.profile(True)
+    plan.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        plan.execute_device(din.ptr, dout.ptr, d)
+    plan.synchronize()
+    elapsed = time.perf_counter() - t0
+    prof = plan.profile_read()
+    plan.profile(False)
+    value = args.steps / elapsed
+    ms_per_step = 1e3 * elapsed / args.steps
+    kernels = {k: {"avg_us": 1e3 * ms / max(c, 1), "launches": c} for k, (ms, c) in prof.items()}
+    log(f"[bench] {value:.1f} transforms/s, {ms_per_step * 1e3:.2f} us/step; kernels {kernels}")
+
+    # correctness against the oracle
+    g = dout.download()
+    sys.path.insert(0, ROOT)
+    from oracle import orc
+    o = orc.FMMPlan(N, maxs, M, d).execute(u, d)
+    err_fmm = float(np.max(np.abs(g - o)) / np.max(np.abs(o)))
+    rng = np.random.default_rng(0)
+    nrows = min(args.check_rows, N)
+    rows = np.unique(np.concatenate([np.arange(min(256, nrows)),
+                                     rng.integers(0, N, max(0, nrows - 256))]))
+    ref = orc.direct_dd_rows(u, rows, d)
+    scale = float(np.max(np.abs(o)))
+    err_direct = float(np.max(np.abs(g[rows] - ref)) / scale)
+    tol = 1e-14 * math.log2(N)
+    log(f"[bench] rel err vs oracle FMM {err_fmm:.3e}, vs double-double direct (rows) {err_direct:.3e}")
+
+    # roofline of the dominant kernel
+    dom = "k_down_band"
+    avg_us = kernels[dom]["avg_us"]
+    alg = down_band_bytes(info, N, d)
+    achieved = alg / (avg_us * 1e-6) / 1e9
+    workload = f"l2c_fp64_N{N}_s{info['s']}_M{M}" if d == 0 else f"c2l_fp64_N{N}_s{info['s']}_M{M}"
+    roofline = {"bound": "hbm", "kernel": dom, "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
+                "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
+                "traffic": pmc_traffic(workload), "algorithmic_bytes": alg,
+                "avg_launch_us": round(avg_us, 3)}
+
+    cpu = None
+    if not args.no_cpu:
+        nt = orc.num_threads()
+        op = orc.FMMPlan(N, maxs, M, d)
+        out = np.zeros(N)
+        op.execute(u, d, out=out)
+        reps, t0 = 0, time.perf_counter()
+        while True:
+            op.execute(u, d, out=out)
+            reps += 1
+            if time.perf_counter() - t0 > args.cpu_seconds:
+                break
+        cpu_t = time.perf_counter() - t0
+        cpu = {"value": round(reps / cpu_t, 3), "unit": "transforms/s", "cores": nt, "kind": "port",
+               "sample": f"oracle FMM execute, N={N} L2C maxs={maxs} M={M}, {reps} repeats "
+                         f"in {cpu_t:.1f} s, OpenMP {nt} threads, own loops (no BLAS)"}
+        log(f"[bench] cpu baseline {cpu}")
+
+    line = {
+        "metric": METRIC, "value": round(value, 2), "unit": "transforms/s", "n_gpus": 1,
+        "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(ms_per_step, 5),
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f64",
+        "data": "synthetic (SplitMix64 U(-1,1) coefficients, seed 1)",
+        "config": {"workload": workload, "N": N, "maxs": maxs, "s": info["s"], "L": info["L"],
+                   "M": M, "direction": "L2C" if d == 0 else "C2L", "form": "plain-fp64",
+                   "inputs_resident": "HBM", "plan_seconds": round(plan_s, 4)},
+        "max_rel_err": {"vs_oracle_fmm": err_fmm, "vs_dd_direct_rows": err_direct,
+                        "rows_checked": int(rows.size), "tol": tol},
+        "kernels_us": {k: round(v["avg_us"], 3) for k, v in kernels.items()},
+        "roofline": roofline,
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(line), flush=True)
+
+
+if __name__ == "__main__":
+    main()

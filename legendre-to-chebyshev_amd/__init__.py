@@ -91,6 +91,10 @@ def lib():
         L.l2c_execute_lines_device.argtypes = [vp, vp, vp, _sz, _sz, _sz, _sz, _sz, _sz, vp]
         L.l2c_set_precision.restype = C.c_int
         L.l2c_set_precision.argtypes = [vp, C.c_int]
+        L.l2c_profile.restype = C.c_int
+        L.l2c_profile.argtypes = [vp, C.c_int]
+        L.l2c_profile_read.restype = C.c_int
+        L.l2c_profile_read.argtypes = [vp, C.POINTER(C.c_double), C.POINTER(_sz)]
         L.l2c_synchronize.restype = C.c_int
         L.l2c_synchronize.argtypes = [vp]
         L.l2c_malloc_device.restype = vp
@@ -107,12 +111,6 @@ def lib():
 
 def last_error():
     return lib().l2c_last_error().decode()
-
-
-def _check(ret, what):
-    if ret is None or ret == _SIZE_MAX or (isinstance(ret, int) and ret < 0 and ret != _SIZE_MAX and what.endswith("rc")):
-        raise L2CError(f"{what}: {last_error()}")
-    return ret
 
 
 def _ptr(a):
@@ -223,6 +221,20 @@ class FMMPlan:
     def synchronize(self):
         if lib().l2c_synchronize(self._plan) != 0:
             raise L2CError(last_error())
+
+    KERNELS = ("k_up_subtree", "k_up_top", "k_down_band", "k_direct", "k_batch",
+               "k_split_m2l", "k_lines_direct", "reserved")
+
+    def profile(self, enable=True):
+        if lib().l2c_profile(self._plan, 1 if enable else 0) != 0:
+            raise L2CError(last_error())
+
+    def profile_read(self):
+        ms = (C.c_double * 8)()
+        cnt = (_sz * 8)()
+        if lib().l2c_profile_read(self._plan, ms, cnt) != 0:
+            raise L2CError(last_error())
+        return {k: (ms[i], cnt[i]) for i, k in enumerate(self.KERNELS) if cnt[i]}
 
     def info(self):
         inf = PlanInfo()

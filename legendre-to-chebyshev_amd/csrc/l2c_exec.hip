@@ -1,17 +1,8 @@
-// l2c_exec.hip -- execute path of the gfx950 FMM Legendre<->Chebyshev
-// transform (src/C/leg2cheb.c:1299-1500 and the near field :236-535).
-//
-// One transform (or one line of a batch) runs as three launches:
-//   k_up_subtree   leaf moments (step 1) and the upward pass (step 2) inside
-//                  a subtree of 2^gu leaf chunks, both parities; writes the
-//                  multipole store w for every level >= lu.
-//   k_up_top       multipoles of the levels above lu, each from its level-lu
-//                  descendants through composite operators P (no level loop).
-//   k_down_band    per subtree of 2^gd leaf chunks: M2L (step 3) and L2L
-//                  (step 4) along its ancestor chain and inside the subtree,
-//                  leaf evaluation (step 5), the near-field band, the final
-//                  scaling, one coalesced store of the output.
-// Both parities share each M2L matrix read.  The output is overwritten.
+// l2c_exec.hip -- C ABI entry points of the gfx950 FMM Legendre<->Chebyshev
+// library: execute / direct / directM / execute2D and the l2c_* extensions
+// (src/C/leg2cheb.c:152-234, 236-376, 1267-1500), the reference's flop
+// formulas, and per-launch HIP-event timing.  The kernels live in
+// l2c_fmm.hip.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -24,403 +15,39 @@
 
 namespace l2c {
 
-struct LineMap {
-  size_t n0, ls0, ls1, es;  // line l starts at (l/n0)*ls1 + (l%n0)*ls0; element i at +i*es
-};
-__device__ inline size_t line_off(const LineMap &m, size_t l) {
-  return (l / m.n0) * m.ls1 + (l % m.n0) * m.ls0;
-}
-
-constexpr double kSqrtPi = 1.77245385090551602729816748334;
-constexpr double k2Pi = 0.63661977236758134307553505349;  // 2/pi
-
 // ---------------------------------------------------------------------------
-// k_up_subtree: grid (2^(lu+2), lines), 256 threads.
-__global__ void __launch_bounds__(256) k_up_subtree(const double *__restrict__ in, LineMap lm,
-                                                    Geo g, int dir,
-                                                    const double *__restrict__ T,
-                                                    const double *__restrict__ U,
-                                                    double *__restrict__ wall, size_t wstride) {
-  extern __shared__ double sm[];
-  const int s = g.s, M = g.M, L = g.L;
-  const int G = 1 << g.gu;
-  const int j = blockIdx.x;
-  const int tid = threadIdx.x, bd = blockDim.x;
-  const double *src = in + line_off(lm, blockIdx.y);
-  double *w = wall + blockIdx.y * wstride;
-  double *xs = sm;               // [2][G*s]
-  double *w0 = xs + 2 * G * s;   // [G][2][M]
-  double *w1 = w0 + G * 2 * M;   // [G/2][2][M]
-  const double *Ts = T, *Us = U;  // small, cache resident
-  const size_t base = (size_t)j * G * 2 * s;
-  for (int e = tid; e < G * 2 * s; e += bd) {
-    const size_t i = base + e;
-    double v = 0;
-    if (i < (size_t)g.N) {
-      v = src[i * lm.es];
-      if (dir == C2L && i > 0) v *= (double)i;  // v_i = i u_i (src/C/leg2cheb.c:1325-1335)
-    }
-    xs[(e & 1) * G * s + (e >> 1)] = v;
-  }
-  __syncthreads();
-  // step 1: moments of each leaf chunk, w[k] = sum_m T_par[m][k] x_par[m]
-  for (int o = tid; o < G * 2 * M; o += bd) {
-    const int c = o / (2 * M), sg = (o / M) & 1, k = o % M;
-    const double *x = xs + sg * G * s + c * s;
-    const double *t = Ts + sg * s * M + k;
-    double acc = 0;
-    for (int m = 0; m < s; m++) acc += t[m * M] * x[m];
-    w0[o] = acc;
-    w[(slot_base(L - 1) + (size_t)j * G + c) * 2 * M + sg * M + k] = acc;
-  }
-  __syncthreads();
-  // step 2: parent = U0 child0 + U1 child1, level by level inside the subtree
-  int n = G;
-  for (int lev = L - 2; lev >= g.lu; lev--) {
-    n >>= 1;
-    for (int o = tid; o < n * 2 * M; o += bd) {
-      const int c = o / (2 * M), sg = (o / M) & 1, k = o % M;
-      const double *x0 = w0 + (2 * c * 2 + sg) * M;
-      const double *x1 = w0 + ((2 * c + 1) * 2 + sg) * M;
-      const double *u0 = Us + k * M, *u1 = Us + M * M + k * M;
-      double acc = 0;
-      for (int l = 0; l < M; l++) acc += u0[l] * x0[l] + u1[l] * x1[l];
-      w1[o] = acc;
-      w[(slot_base(lev) + (size_t)j * n + c) * 2 * M + sg * M + k] = acc;
-    }
-    __syncthreads();
-    double *t = w0;
-    w0 = w1;
-    w1 = t;
-  }
-}
-
-// ---------------------------------------------------------------------------
-// k_up_top: grid (slot_base(lu), lines), 256 threads.  Multipole of chunk c
-// at level l < lu = sum_d P(lu-l, d) w[lu][c 2^(lu-l) + d].
-__global__ void __launch_bounds__(256) k_up_top(Geo g, const double *__restrict__ P,
-                                                double *__restrict__ wall, size_t wstride) {
-  __shared__ double red[256];
-  const int M = g.M, MM = M * M, M2 = 2 * M;
-  double *w = wall + blockIdx.y * wstride;
-  const int q = blockIdx.x;
-  int lev = 0;
-  while ((int)slot_base(lev + 1) <= q) lev++;
-  const int c = q - (int)slot_base(lev);
-  const int k = g.lu - lev, nd = 1 << k;
-  const double *Pk = P + (size_t)((1 << k) - 2) * MM;
-  const double *child = w + (slot_base(g.lu) + (size_t)c * nd) * M2;
-  const int tid = threadIdx.x, ngrp = blockDim.x / M2;
-  const int o = tid % M2, grp = tid / M2;
-  double acc = 0;
-  if (grp < ngrp) {
-    const int sg = o / M, i = o % M;
-    for (int d = grp; d < nd; d += ngrp) {
-      const double *pr = Pk + (size_t)d * MM + i * M;
-      const double *x = child + (size_t)d * M2 + sg * M;
-      for (int l = 0; l < M; l++) acc += pr[l] * x[l];
+// per-launch event timing
+void prof_begin(Prof *pf, int id, hipStream_t st) {
+  if (!pf || !pf->on) return;
+  if (pf->used + 2 > pf->pool.size()) {
+    for (int i = 0; i < 256; i++) {
+      hipEvent_t e;
+      if (hipEventCreate(&e) != hipSuccess) return;
+      pf->pool.push_back(e);
     }
   }
-  red[tid] = acc;
-  __syncthreads();
-  if (tid < M2) {
-    double sum = 0;
-    for (int gi = 0; gi < ngrp; gi++) sum += red[gi * M2 + tid];
-    w[(slot_base(lev) + c) * M2 + tid] = sum;
-  }
+  Prof::Rec r{id, pf->pool[pf->used], pf->pool[pf->used + 1]};
+  pf->used += 2;
+  (void)hipEventRecord(r.a, st);
+  pf->recs.push_back(r);
 }
 
-// M2L for row chunk r at `level` (src/C/leg2cheb.c:1419-1435): block b = r/2;
-// r even: A(b,0) w[r+2] + A(b,1) w[r+3]; r odd: A(b,2) w[r+2].
-__device__ inline double m2l_row(const double *__restrict__ A, const double *__restrict__ w,
-                                 int level, size_t r, int sg, int k, int M) {
-  const size_t MM = (size_t)M * M;
-  const double *Ab = A + (3 * total_blocks(level) + 3 * (r >> 1)) * MM;
-  const double *wb = w + (slot_base(level) + r + 2) * 2 * M + sg * M;
-  double acc = 0;
-  if ((r & 1) == 0) {
-    const double *a0 = Ab + k * M, *a1 = Ab + MM + k * M;
-    double acc1 = 0;
-    for (int l = 0; l < M; l++) acc += a0[l] * wb[l];
-    for (int l = 0; l < M; l++) acc1 += a1[l] * wb[2 * M + l];
-    acc += acc1;
-  } else {
-    const double *a2 = Ab + 2 * MM + k * M;
-    for (int l = 0; l < M; l++) acc += a2[l] * wb[l];
-  }
-  return acc;
+void prof_end(Prof *pf, hipStream_t st) {
+  if (!pf || !pf->on || pf->recs.empty()) return;
+  (void)hipEventRecord(pf->recs.back().b, st);
 }
 
-// ---------------------------------------------------------------------------
-// k_down_band: grid (2^(ld+2), lines), 256 threads.
-__global__ void __launch_bounds__(256) k_down_band(
-    const double *__restrict__ in, double *__restrict__ out, LineMap lm, Geo g, int dir,
-    int band_only, int precompute, const double *__restrict__ A,
-    const double *__restrict__ wall, size_t wstride, const double *__restrict__ D,
-    const double *__restrict__ T, const double *__restrict__ a, const double *__restrict__ an,
-    const double *__restrict__ dn) {
-  extern __shared__ double sm[];
-  const int s = g.s, M = g.M, L = g.L, N = g.N, M2 = 2 * M;
-  const int G = 1 << g.gd;
-  const int R = blockIdx.x;
-  const int tid = threadIdx.x, bd = blockDim.x;
-  const size_t row0 = (size_t)R * G * 2 * s;
-  if (row0 >= (size_t)N) return;
-  const size_t row1 = std::min((size_t)N, row0 + (size_t)G * 2 * s);
-  const size_t off = line_off(lm, blockIdx.y);
-  const double *src = in + off;
-  double *dst = out + off;
-  const double *w = wall + blockIdx.y * wstride;
-
-  // LDS carve-up
-  double *cA = sm;                     // [G][2][M]
-  double *cB = cA + G * M2;            // [G][2][M]
-  double *chain = cB + G * M2;         // [ld+1][2][M]
-  double *xw = chain + (g.ld + 1) * M2;  // input window, G*2s + 2s
-  const double *Ds = D, *Tl = T;       // small, cache resident
-  double *aw = xw + G * 2 * s + 2 * s; // Lambda window (a or an), G*2s + 2s
-  double *hd = aw + G * 2 * s + 2 * s; // a[0..2s) or dn[0..2s)
-  double *cur = cA;
-
-  // input window [row0, min(N, row1 + 2s)) with the C2L pre-scale
-  const size_t wend = std::min((size_t)N, row1 + 2 * (size_t)s);
-  for (size_t e = row0 + tid; e < wend; e += bd) {
-    double v = src[e * lm.es];
-    if (dir == C2L && e > 0) v *= (double)e;
-    xw[e - row0] = v;
-  }
-  const double *tab = dir == L2C ? a : an;
-  for (size_t e = row0 + tid; e < wend; e += bd) aw[e - row0] = tab[e];
-  const double *htab = dir == L2C ? a : dn;
-  const int nh = std::min(2 * s, dir == L2C ? N : (N + 1) / 2);
-  for (int e = tid; e < 2 * s; e += bd) hd[e] = e < nh ? htab[e] : 0.0;
-
-  if (!band_only) {
-    // M2L along the ancestor chain, levels 0..ld (level ld is chunk R itself)
-    for (int o = tid; o < (g.ld + 1) * M2; o += bd) {
-      const int lev = o / M2, sg = (o % M2) / M, k = o % M;
-      const size_t r = (size_t)R >> (g.ld - lev);
-      chain[o] = r < ((size_t)4 << lev) - 2 ? m2l_row(A, w, lev, r, sg, k, M) : 0.0;
-    }
-    __syncthreads();
-    // L2L down the chain: c_l = D(p) c_{l-1} + m2l_l (src/C/leg2cheb.c:1437-1449)
-    for (int lev = 1; lev <= g.ld; lev++) {
-      const size_t r = (size_t)R >> (g.ld - lev);
-      const bool valid = r < ((size_t)4 << lev) - 2;
-      const int p = (int)(r & 1);
-      double v = 0;
-      if (tid < M2) {
-        const int sg = tid / M, i = tid % M;
-        const double *par = chain + (lev - 1) * M2 + sg * M;
-        const double *dr = Ds + p * M * M + i * M;
-        double acc = 0;
-        for (int jj = 0; jj < M; jj++) acc += dr[jj] * par[jj];
-        v = valid ? acc + chain[lev * M2 + tid] : 0.0;
-      }
-      __syncthreads();
-      if (tid < M2) chain[lev * M2 + tid] = v;
-      __syncthreads();
-    }
-    for (int e = tid; e < M2; e += bd) cA[e] = chain[g.ld * M2 + e];
-    __syncthreads();
-    // inside the subtree: levels ld+1 .. L-1
-    double *nxt = cB;
-    for (int lev = g.ld + 1; lev < L; lev++) {
-      const int n = 1 << (lev - g.ld);
-      for (int o = tid; o < n * M2; o += bd) {
-        const int gi = o / M2, sg = (o % M2) / M, k = o % M;
-        const size_t r = (size_t)R * n + gi;
-        double v = 0;
-        if (r < ((size_t)4 << lev) - 2) {
-          const double *par = cur + (gi >> 1) * M2 + sg * M;
-          const double *dr = Ds + (int)(r & 1) * M * M + k * M;
-          double acc = 0;
-          for (int jj = 0; jj < M; jj++) acc += dr[jj] * par[jj];
-          v = acc + m2l_row(A, w, lev, r, sg, k, M);
-        }
-        nxt[o] = v;
-      }
-      __syncthreads();
-      double *t = cur;
-      cur = nxt;
-      nxt = t;
+static void prof_harvest(Prof *pf) {
+  for (auto &r : pf->recs) {
+    (void)hipEventSynchronize(r.b);
+    float ms = 0;
+    if (hipEventElapsedTime(&ms, r.a, r.b) == hipSuccess) {
+      pf->ms[r.id] += ms;
+      pf->cnt[r.id]++;
     }
   }
-  __syncthreads();
-
-  // near field + leaf evaluation + final scaling, one row per thread
-  const size_t nLm1 = (size_t)g.nL - 1;
-  const size_t nvalid = ((size_t)2 << L) - 2;  // leaf chunks with a local expansion
-  for (size_t i = row0 + tid; i < row1; i += bd) {
-    const size_t beta = i / (2 * s);
-    const size_t jend = beta < nLm1 ? 2 * (size_t)s * (beta + 2) : (size_t)N;
-    double acc = 0;
-    if (dir == L2C) {
-      for (size_t jx = i; jx < jend; jx += 2) {
-        const size_t h = (jx - i) >> 1, pp = (jx + i) >> 1;
-        acc += hd[h] * aw[pp - row0] * xw[jx - row0];
-      }
-    } else {
-      acc = kSqrtPi * xw[i - row0] * aw[i - row0];
-      for (size_t jx = i + 2; jx < jend; jx += 2) {
-        const size_t h = (jx - i) >> 1, pp = (jx + i) >> 1;
-        if (precompute)
-          acc -= hd[h] * aw[pp - row0] * xw[jx - row0];
-        else
-          acc -= hd[h] * xw[jx - row0] * aw[pp - row0];
-      }
-    }
-    double far = 0;
-    if (!band_only) {
-      const size_t r = i >> 1;
-      const int sg = (int)(i & 1);
-      const size_t t = r / s;
-      if (t < nvalid) {
-        const int m = (int)(r - t * s);
-        const int gi = (int)(t - (size_t)R * G);
-        const double *c = cur + gi * M2 + sg * M;
-        const double *tr = Tl + (sg * s + m) * M;
-        for (int k = 0; k < M; k++) far += tr[k] * c[k];
-      }
-    }
-    double v;
-    if (dir == L2C) {
-      v = (far + acc) * k2Pi;
-      if (i == 0) v *= 0.5;
-    } else {
-      v = (far + acc) * ((double)i + 0.5);
-    }
-    dst[i * lm.es] = v;
-  }
-}
-
-size_t down_shared_bytes(const Geo &g) {
-  const int G = 1 << g.gd;
-  return sizeof(double) * ((size_t)2 * G * 2 * g.M + (g.ld + 1) * 2 * g.M +
-                           2 * ((size_t)G * 2 * g.s + 2 * g.s) + 2 * g.s);
-}
-size_t up_shared_bytes(const Geo &g) {
-  const int G = 1 << g.gu;
-  return sizeof(double) * ((size_t)2 * G * g.s + G * 2 * g.M + (G / 2 + 1) * 2 * g.M);
-}
-
-// ---------------------------------------------------------------------------
-// direct O(N^2) sums, one row per thread: src/C/leg2cheb.c:152-234
-__global__ void __launch_bounds__(256) k_direct(const double *__restrict__ in,
-                                                double *__restrict__ out, LineMap lm, int N,
-                                                int dir, const double *__restrict__ a,
-                                                const double *__restrict__ an,
-                                                const double *__restrict__ dn) {
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= N) return;
-  const size_t off = line_off(lm, blockIdx.y);
-  const double *u = in + off;
-  const size_t es = lm.es;
-  double acc = 0;
-  if (dir == L2C) {
-    for (int n = 0; n + i < N; n += 2) acc += a[n / 2] * k2Pi * a[n / 2 + i] * u[(size_t)(n + i) * es];
-    if (i == 0) acc /= 2;
-  } else {
-    const double vi = i == 0 ? u[0] : u[(size_t)i * es] * (double)i;
-    acc = kSqrtPi * vi * an[i];
-    for (int n = 2; n + i < N; n += 2)
-      acc -= dn[n / 2] * an[n / 2 + i] * (u[(size_t)(n + i) * es] * (double)(n + i));
-    acc *= ((double)i + 0.5);
-  }
-  out[off + (size_t)i * es] = acc;
-}
-
-// ---------------------------------------------------------------------------
-// host orchestration
-
-// Multipole store for a batch of lines: grown on demand, owned by the plan.
-static int ensure_lines_ws(fmm_plan_s *p, size_t nlines, double **w) {
-  if (nlines <= 1) {
-    *w = p->d_w;
-    return 0;
-  }
-  const size_t need = nlines * slot_base(p->geo.L) * 2 * p->M;
-  if (p->wl_cap < need) {
-    L2C_HIP(hipStreamSynchronize(p->stream), -1);
-    if (p->d_wl) (void)hipFree(p->d_wl);
-    p->d_wl = nullptr;
-    p->wl_cap = 0;
-    L2C_HIP(hipMalloc(&p->d_wl, need * sizeof(double)), -1);
-    p->wl_cap = need;
-  }
-  *w = p->d_wl;
-  return 0;
-}
-
-static int launch_fmm(fmm_plan_s *p, const double *d_in, double *d_out, size_t dir,
-                      const LineMap &lm, size_t nlines, hipStream_t st, int band_only,
-                      double *w) {
-  const Geo &g = p->geo;
-  const size_t wstride = slot_base(g.L) * 2 * g.M;
-  if (!band_only) {
-    dim3 gu((unsigned)((size_t)4 << g.lu), (unsigned)nlines);
-    k_up_subtree<<<gu, 256, up_shared_bytes(g), st>>>(d_in, lm, g, (int)dir, p->t.T, p->t.U, w,
-                                                     wstride);
-    L2C_HIP(hipGetLastError(), -1);
-    if (g.lu > 0) {
-      dim3 gt((unsigned)slot_base(g.lu), (unsigned)nlines);
-      k_up_top<<<gt, 256, 0, st>>>(g, p->t.P, w, wstride);
-      L2C_HIP(hipGetLastError(), -1);
-    }
-  }
-  dim3 gd((unsigned)((size_t)4 << g.ld), (unsigned)nlines);
-  k_down_band<<<gd, 256, down_shared_bytes(g), st>>>(
-      d_in, d_out, lm, g, (int)dir, band_only, (int)p->precompute, p->t.A[dir], w, wstride,
-      p->t.D, p->t.T, p->t.a, p->t.an, p->t.dn);
-  L2C_HIP(hipGetLastError(), -1);
-  return 0;
-}
-
-int run_fmm(fmm_plan_s *p, const double *d_in, double *d_out, size_t dir, size_t stride,
-            hipStream_t st, int band_only) {
-  LineMap lm{1, 0, 0, stride};
-  return launch_fmm(p, d_in, d_out, dir, lm, 1, st, band_only, p->d_w);
-}
-
-int run_direct(const direct_plan_s *dp, const double *d_in, double *d_out, size_t dir,
-               size_t stride, hipStream_t st, size_t precompute) {
-  (void)precompute;
-  LineMap lm{1, 0, 0, stride};
-  const int N = (int)dp->N;
-  k_direct<<<dim3((N + 255) / 256, 1), 256, 0, st>>>(d_in, d_out, lm, N, (int)dir, dp->d_a,
-                                                     dp->d_an, dp->d_dn);
-  L2C_HIP(hipGetLastError(), -1);
-  return 0;
-}
-
-int run_lines(fmm_plan_s *p, const double *d_in, double *d_out, size_t dir, size_t nlines,
-              size_t n0, size_t ls0, size_t ls1, size_t es, hipStream_t st) {
-  LineMap lm{n0 ? n0 : 1, ls0, ls1, es};
-  if (p->L == 0) {
-    const direct_plan_s *dp = p->dplan;
-    const int N = (int)dp->N;
-    for (size_t l0 = 0; l0 < nlines; l0 += 65535) {
-      const size_t nl = std::min<size_t>(65535, nlines - l0);
-      // line offsets are computed from blockIdx.y: shift the map by l0 lines
-      LineMap lml = lm;
-      const size_t base = (l0 / lm.n0) * lm.ls1 + (l0 % lm.n0) * lm.ls0;
-      if (l0 % lm.n0 != 0) {
-        set_error("run_lines: batches above 65535 lines need n0 to divide 65535-line chunks");
-        return -1;
-      }
-      k_direct<<<dim3((N + 255) / 256, (unsigned)nl), 256, 0, st>>>(
-          d_in + base, d_out + base, lml, N, (int)dir, dp->d_a, dp->d_an, dp->d_dn);
-      L2C_HIP(hipGetLastError(), -1);
-    }
-    return 0;
-  }
-  if (nlines > 65535) {
-    set_error("run_lines: at most 65535 lines per call for FMM plans");
-    return -1;
-  }
-  double *w = nullptr;
-  if (ensure_lines_ws(p, nlines, &w)) return -1;
-  return launch_fmm(p, d_in, d_out, dir, lm, nlines, st, 0, w);
+  pf->recs.clear();
+  pf->used = 0;
 }
 
 // ---------------------------------------------------------------------------
@@ -624,6 +251,35 @@ int l2c_execute_lines_device(fmm_plan *p, const double *d_in, double *d_out, siz
   }
   hipStream_t st = stream ? (hipStream_t)stream : p->stream;
   return run_lines(p, d_in, d_out, direction, nlines, n0, ls0, ls1, es, st);
+}
+
+int l2c_profile(fmm_plan *p, int enable) {
+  if (!p) {
+    set_error("l2c_profile: NULL plan");
+    return -1;
+  }
+  if (!p->prof) p->prof = new Prof();
+  if (p->prof->on) prof_harvest(p->prof);
+  p->prof->on = enable;
+  for (int i = 0; i < K_COUNT; i++) {
+    p->prof->ms[i] = 0;
+    p->prof->cnt[i] = 0;
+  }
+  if (p->dplan) p->dplan->prof = p->prof;
+  return 0;
+}
+
+int l2c_profile_read(fmm_plan *p, double *ms, size_t *counts) {
+  if (!p || !p->prof) {
+    set_error("l2c_profile_read: profiling not enabled");
+    return -1;
+  }
+  prof_harvest(p->prof);
+  for (int i = 0; i < K_COUNT; i++) {
+    if (ms) ms[i] = p->prof->ms[i];
+    if (counts) counts[i] = p->prof->cnt[i];
+  }
+  return 0;
 }
 
 // 2-D driver: src/C/leg2cheb.c:1236-1297, every line of an axis in one launch.

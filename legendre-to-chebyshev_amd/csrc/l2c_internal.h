@@ -7,6 +7,7 @@
 #include <cstddef>
 #include <cstdint>
 #include <string>
+#include <vector>
 
 #include "../../include/leg2cheb_amd.h"
 
@@ -82,11 +83,21 @@ __host__ __device__ inline size_t total_sub(size_t L) { return 3 * total_blocks(
 // Chunk slots of the multipole store: level l holds 2^(l+2) chunks.
 __host__ __device__ inline size_t slot_base(int level) { return ((size_t)4 << level) - 4; }
 
+// Line l of a batch starts at (l / n0) * ls1 + (l % n0) * ls0; its element i
+// is at that offset + i * es.  A single transform is {1, 0, 0, stride}.
+struct LineMap {
+  size_t n0, ls0, ls1, es;
+};
+__host__ __device__ inline size_t line_off(const LineMap &m, size_t l) {
+  return (l / m.n0) * m.ls1 + (l % m.n0) * m.ls0;
+}
+
 struct Geo {
   int N, Nn, L, s, M;
   int nL;      // N / (2s): number of 2s-row near-field blocks
   int gu, lu;  // upward work-group: 2^gu leaf chunks; cut level lu = L-1-gu
   int gd, ld;  // downward work-group: 2^gd leaf chunks; cut level ld = L-1-gd
+  int nodal;   // 1: Lagrange basis (lagrange=1), 0: Chebyshev basis
 };
 
 struct DevTables {
@@ -101,6 +112,22 @@ struct DevTables {
   double *Asl[2] = {nullptr, nullptr};  // split-precision M2L slices
 };
 
+// Per-launch HIP-event timing of the execute kernels (bench support).
+enum KernelId { K_UP_SUBTREE = 0, K_UP_TOP = 1, K_DOWN_BAND = 2, K_DIRECT = 3, K_COUNT = 8 };
+struct Prof {
+  int on = 0;
+  struct Rec {
+    int id;
+    hipEvent_t a, b;
+  };
+  std::vector<Rec> recs;
+  std::vector<hipEvent_t> pool;
+  size_t used = 0;
+  double ms[K_COUNT] = {0};
+  size_t cnt[K_COUNT] = {0};
+};
+void prof_begin(Prof *pf, int id, hipStream_t st);
+void prof_end(Prof *pf, hipStream_t st);
 }  // namespace l2c
 
 struct direct_plan_s {
@@ -110,6 +137,7 @@ struct direct_plan_s {
   double *d_in = nullptr, *d_out = nullptr;  // staging (device)
   double *h_in = nullptr, *h_out = nullptr;  // staging (pinned host)
   hipStream_t stream = nullptr;
+  l2c::Prof *prof = nullptr;
 };
 
 struct fmm_plan_s {
@@ -127,6 +155,7 @@ struct fmm_plan_s {
   double *h_in = nullptr, *h_out = nullptr;  // staging (pinned host)
   size_t hbm_bytes = 0;
   hipStream_t stream = nullptr;
+  l2c::Prof *prof = nullptr;
 };
 
 struct fmm_plan_2d_s {
@@ -147,6 +176,7 @@ int run_lines(fmm_plan_s *p, const double *d_in, double *d_out, size_t dir, size
 size_t flops_fmm(const fmm_plan_s *p, size_t dir);
 size_t down_shared_bytes(const Geo &g);
 size_t up_shared_bytes(const Geo &g);
+size_t ws_doubles(const Geo &g);  // execute workspace per line
 size_t flops_direct(size_t N, size_t dir, size_t precompute);
 // implemented in l2c_plan.hip
 direct_plan_s *make_direct(size_t N, size_t direction, size_t precompute, bool staging);

@@ -342,35 +342,41 @@ static void destroy_fmm(fmm_plan_s *p) {
   dfree(p->d_out);
   if (p->h_in) (void)hipHostFree(p->h_in);
   if (p->h_out) (void)hipHostFree(p->h_out);
+  if (p->dplan) p->dplan->prof = nullptr;
   destroy_direct(p->dplan);
+  if (p->prof) {
+    for (auto e : p->prof->pool) (void)hipEventDestroy(e);
+    delete p->prof;
+  }
   if (p->stream) (void)hipStreamDestroy(p->stream);
   delete p;
 }
 
 // Work-group shapes: leaf chunks per upward / downward work-group, shrunk
 // until the work-group's LDS fits the budget.
-static void choose_cuts(Geo &g) {
+static bool choose_cuts(Geo &g) {
   const int Lm1 = g.L - 1;
   int gu = 4, gd = 3;
   if (const char *e = getenv("L2C_GU")) gu = atoi(e);
   if (const char *e = getenv("L2C_GD")) gd = atoi(e);
-  const size_t budget = 56 * 1024;
+  const size_t budget = 64 * 1024;
   g.gu = std::max(0, std::min(gu, Lm1));
-  g.gd = std::max(0, std::min(gd, Lm1));
+  g.gd = std::max(0, std::min(gd, g.gu));  // k_down reads multipoles below lu only
   for (;;) {
     g.lu = Lm1 - g.gu;
     g.ld = Lm1 - g.gd;
     bool shrink = false;
     if (up_shared_bytes(g) > budget && g.gu > 0) {
       g.gu--;
+      if (g.gd > g.gu) g.gd = g.gu;
       shrink = true;
-    }
-    if (down_shared_bytes(g) > budget && g.gd > 0) {
+    } else if (down_shared_bytes(g) > budget && g.gd > 0) {
       g.gd--;
       shrink = true;
     }
     if (!shrink) break;
   }
+  return up_shared_bytes(g) <= budget && down_shared_bytes(g) <= budget;
 }
 
 fmm_plan_s *create_fmm_impl(size_t N, size_t maxs, size_t M, size_t direction, size_t lagrange,
@@ -434,7 +440,12 @@ fmm_plan_s *create_fmm_impl(size_t N, size_t maxs, size_t M, size_t direction, s
   g.s = (int)s;
   g.M = (int)M;
   g.nL = (int)(N / (2 * s));
-  choose_cuts(g);
+  g.nodal = lagrange ? 1 : 0;
+  if (!choose_cuts(g)) {
+    set_error("create_fmm: leaf size s and rank M need more than 64 KiB of LDS per work-group");
+    destroy_fmm(p);
+    return nullptr;
+  }
   if (v > 1) {
     printf("N %zu\nNum levels %d\nNum submatrices %zu\nNum blocks %zu\n", N, L, p->Nc, p->Nb);
     printf("Given max s %zu \nComputed s %zu \nComputed N %zu\nLagrange %zu\n", maxs, s, Nn,
@@ -575,7 +586,9 @@ fmm_plan_s *create_fmm_impl(size_t N, size_t maxs, size_t M, size_t direction, s
 
   // multipole store and staging
   const size_t slots = slot_base(L);
-  if (dmalloc(&p->d_w, slots * 2 * M, acct) || dmalloc(&p->d_in, N, acct) ||
+  (void)slots;
+  p->wl_cap = ws_doubles(g);
+  if (dmalloc(&p->d_wl, p->wl_cap, acct) || dmalloc(&p->d_in, N, acct) ||
       dmalloc(&p->d_out, N, acct)) {
     destroy_fmm(p);
     return nullptr;

@@ -73,6 +73,8 @@ def lib():
         L.orc_execute2D.argtypes = [_dp, _dp, C.c_void_p, _sz]
         for f in ("orc_leg2cheb_q", "orc_cheb2leg_q", "orc_leg2cheb_dd", "orc_cheb2leg_dd"):
             getattr(L, f).argtypes = [_dp, _dp, _sz, _sz, _sz]
+        for f in ("orc_leg2cheb_dd_rows", "orc_cheb2leg_dd_rows"):
+            getattr(L, f).argtypes = [_dp, _dp, _sz, C.POINTER(_sz), _sz]
         L.orc_num_threads.restype = C.c_int
         L.orc_set_num_threads.argtypes = [C.c_int]
         _lib = L
@@ -218,6 +220,16 @@ def leg2cheb_dd(u, r0=0, r1=None):
 
 def cheb2leg_dd(u, r0=0, r1=None):
     return _rows(lib().orc_cheb2leg_dd, u, r0, r1)
+
+
+def direct_dd_rows(u, rows, direction):
+    """Selected rows of the direct sum (double-double), direction 0 L2C / 1 C2L."""
+    u = np.ascontiguousarray(u, dtype=np.float64)
+    r = np.ascontiguousarray(rows, dtype=np.uint64)
+    out = np.empty(r.size)
+    f = lib().orc_leg2cheb_dd_rows if direction == 0 else lib().orc_cheb2leg_dd_rows
+    f(_p(u), _p(out), u.size, r.ctypes.data_as(C.POINTER(_sz)), r.size)
+    return out
 
 
 def num_threads():

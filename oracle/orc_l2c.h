@@ -113,6 +113,8 @@ void orc_leg2cheb_q(const double *u, double *b, size_t N, size_t r0, size_t r1);
 void orc_cheb2leg_q(const double *u, double *b, size_t N, size_t r0, size_t r1);
 void orc_leg2cheb_dd(const double *u, double *b, size_t N, size_t r0, size_t r1);
 void orc_cheb2leg_dd(const double *u, double *b, size_t N, size_t r0, size_t r1);
+void orc_leg2cheb_dd_rows(const double *u, double *b, size_t N, const size_t *rows, size_t nr);
+void orc_cheb2leg_dd_rows(const double *u, double *b, size_t N, const size_t *rows, size_t nr);
 /* Lambda(x) in __float128 via lgammaq, rounded to double (pins orc_lambda). */
 double orc_lambda_q(double x);
 

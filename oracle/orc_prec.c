@@ -105,24 +105,50 @@ static inline dd dd_mul_d(dd a, double b) {
 }
 static inline double dd_to_d(dd a) { return a.hi + a.lo; }
 
-void orc_leg2cheb_dd(const double *u, double *b, size_t N, size_t r0, size_t r1) {
+/* rows[k] (k < nr) of the L2C / C2L direct sums in double-double. */
+static void l2c_dd_rows(const double *u, double *b, size_t N, const size_t *rows, size_t nr) {
   __float128 *aq = lam_table_q(N);
   dd *a = (dd *)malloc((N ? N : 1) * sizeof(dd));
   for (size_t k = 0; k < N; k++) a[k] = dd_from_q(aq[k]);
   const dd twopi = dd_from_q(2 / M_PIq);
-#pragma omp parallel for schedule(dynamic, 16)
-  for (size_t i = r0; i < r1; i++) {
+#pragma omp parallel for schedule(dynamic, 4)
+  for (size_t k = 0; k < nr; k++) {
+    const size_t i = rows[k];
     dd acc = {0, 0};
     for (size_t n = 0; i + n < N; n += 2) acc = dd_add(acc, dd_mul_d(dd_mul(a[n / 2], a[n / 2 + i]), u[n + i]));
     acc = dd_mul(acc, twopi);
     if (i == 0) acc = dd_mul_d(acc, 0.5);
-    b[i - r0] = dd_to_d(acc);
+    b[k] = dd_to_d(acc);
   }
   free(aq);
   free(a);
 }
 
+static void c2l_dd_rows(const double *u, double *b, size_t N, const size_t *rows, size_t nr);
+
+void orc_leg2cheb_dd(const double *u, double *b, size_t N, size_t r0, size_t r1) {
+  size_t *rows = (size_t *)malloc((r1 > r0 ? r1 - r0 : 1) * sizeof(size_t));
+  for (size_t i = r0; i < r1; i++) rows[i - r0] = i;
+  l2c_dd_rows(u, b, N, rows, r1 - r0);
+  free(rows);
+}
+
+void orc_leg2cheb_dd_rows(const double *u, double *b, size_t N, const size_t *rows, size_t nr) {
+  l2c_dd_rows(u, b, N, rows, nr);
+}
+
+void orc_cheb2leg_dd_rows(const double *u, double *b, size_t N, const size_t *rows, size_t nr) {
+  c2l_dd_rows(u, b, N, rows, nr);
+}
+
 void orc_cheb2leg_dd(const double *u, double *b, size_t N, size_t r0, size_t r1) {
+  size_t *rows = (size_t *)malloc((r1 > r0 ? r1 - r0 : 1) * sizeof(size_t));
+  for (size_t i = r0; i < r1; i++) rows[i - r0] = i;
+  c2l_dd_rows(u, b, N, rows, r1 - r0);
+  free(rows);
+}
+
+static void c2l_dd_rows(const double *u, double *b, size_t N, const size_t *rows, size_t nr) {
   __float128 *aq = lam_table_q(N);
   dd *an = (dd *)malloc((N ? N : 1) * sizeof(dd));
   dd *dn = (dd *)malloc((N / 2 + 2) * sizeof(dd));
@@ -131,8 +157,9 @@ void orc_cheb2leg_dd(const double *u, double *b, size_t N, size_t r0, size_t r1)
   dn[0] = dd_from_q(0);
   for (size_t j = 1; j < N / 2 + 2; j++) dn[j] = dd_from_q(aq[j - 1 < N ? j - 1 : 0] / (2 * (__float128)j));
   const dd sp = dd_from_q(sqrtq(M_PIq));
-#pragma omp parallel for schedule(dynamic, 16)
-  for (size_t n = r0; n < r1; n++) {
+#pragma omp parallel for schedule(dynamic, 4)
+  for (size_t k = 0; k < nr; k++) {
+    const size_t n = rows[k];
     /* v_n = n u_n is exact in double-double */
     dd vn = n == 0 ? (dd){u[0], 0} : dd_mul_d((dd){u[n], 0}, (double)n);
     dd acc = dd_mul(dd_mul(sp, an[n]), vn);
@@ -144,7 +171,7 @@ void orc_cheb2leg_dd(const double *u, double *b, size_t N, size_t r0, size_t r1)
       acc = dd_add(acc, t);
     }
     acc = dd_mul_d(acc, (double)n + 0.5);
-    b[n - r0] = dd_to_d(acc);
+    b[k] = dd_to_d(acc);
   }
   free(aq);
   free(an);
