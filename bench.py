@@ -43,13 +43,24 @@ def log(msg):
     print(msg, file=sys.stderr, flush=True)
 
 
-def down_band_bytes(info, N, direction):
-    """Algorithmic HBM bytes of one k_down_band launch: the M2L matrices
-    (Nc*M^2), input, output, the Lambda table(s) the band reads (a for L2C;
-    an for C2L, dn is a 2s-entry head) and the multipole store (slots*2*M)."""
-    M, L = info["M"], info["L"]
+def kernel_bytes(name, info, N):
+    """Algorithmic HBM bytes of one launch of an execute kernel.
+
+    k_m2l: the M2L matrices (Nc*M^2 doubles), two multipole vectors read and
+    two M2L vectors written per block (Nb blocks, 2M doubles each).
+    k_down: input, output, the Lambda table the band reads (a for L2C, an for
+    C2L; the 2s-entry head is negligible) and one M2L vector per chunk slot.
+    k_up: input, one multipole vector per chunk slot.
+    """
+    M, L, Nb, Nc = info["M"], info["L"], info["Nb"], info["Nc"]
     slots = (4 << L) - 4
-    return 8 * (info["Nc"] * M * M + 3 * N + slots * 2 * M)
+    if name == "k_m2l":
+        return 8 * (Nc * M * M + 4 * Nb * 2 * M)
+    if name == "k_down":
+        return 8 * (3 * N + slots * 2 * M)
+    if name == "k_up":
+        return 8 * (N + slots * 2 * M)
+    return 8 * 2 * N
 
 
 def pmc_traffic(workload):
@@ -132,10 +143,10 @@ def main():
     tol = 1e-14 * math.log2(N)
     log(f"[bench] rel err vs oracle FMM {err_fmm:.3e}, vs double-double direct (rows) {err_direct:.3e}")
 
-    # roofline of the dominant kernel
-    dom = "k_down_band"
+    # roofline of the dominant (longest) kernel
+    dom = max(kernels, key=lambda k: kernels[k]["avg_us"])
     avg_us = kernels[dom]["avg_us"]
-    alg = down_band_bytes(info, N, d)
+    alg = kernel_bytes(dom, info, N)
     achieved = alg / (avg_us * 1e-6) / 1e9
     workload = f"l2c_fp64_N{N}_s{info['s']}_M{M}" if d == 0 else f"c2l_fp64_N{N}_s{info['s']}_M{M}"
     roofline = {"bound": "hbm", "kernel": dom, "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,

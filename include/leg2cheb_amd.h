@@ -152,7 +152,7 @@ int l2c_set_precision(fmm_plan *plan, int form);
 /* Per-kernel timing with HIP events recorded on the launch stream around
  * every execute kernel.  l2c_profile(plan, 1) resets and enables it;
  * l2c_profile_read fills ms[8] (summed milliseconds) and counts[8] (launches)
- * indexed 0 k_up_subtree, 1 k_up_top, 2 k_down_band, 3 k_direct,
+ * indexed 0 k_up, 1 k_m2l, 2 k_down, 3 k_direct,
  * 4 k_batch, 5 k_split_m2l, 6 k_lines_direct, 7 reserved. */
 int l2c_profile(fmm_plan *plan, int enable);
 int l2c_profile_read(fmm_plan *plan, double *ms, size_t *counts);

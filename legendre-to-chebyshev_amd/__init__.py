@@ -222,7 +222,7 @@ class FMMPlan:
         if lib().l2c_synchronize(self._plan) != 0:
             raise L2CError(last_error())
 
-    KERNELS = ("k_up_subtree", "k_up_top", "k_down_band", "k_direct", "k_batch",
+    KERNELS = ("k_up", "k_m2l", "k_down", "k_direct", "k_batch",
                "k_split_m2l", "k_lines_direct", "reserved")
 
     def profile(self, enable=True):

@@ -2,21 +2,24 @@
 // and the near field :236-535), plus the direct O(N^2) kernel (:152-234).
 //
 // One transform (or one line of a batch) is three launches, both parities in
-// every work-group so that each M2L matrix is read once per transform:
+// every work-group:
 //
-//   k_up      leaf moments (step 1) and the upward pass (step 2) inside a
-//             subtree of 2^gu leaf chunks; writes the multipole store for
-//             levels >= lu and, for every level l < lu, the subtree root's
-//             contribution P(lu-l, .) w_root to its level-l ancestor.
-//   k_top     M2L (step 3) of every row chunk above level ld, one work-group
-//             each; multipoles above lu are sums of the k_up contributions.
-//   k_down    per subtree of 2^gd leaf chunks: M2L below ld, L2L (step 4)
-//             down the ancestor chain and the subtree, the near-field band,
-//             leaf evaluation (step 5), the final scaling and one coalesced
-//             store of the output (which is overwritten, not accumulated).
+//   k_up    leaf moments (step 1) on FP64 MFMA and the upward pass (step 2)
+//           inside a subtree of 2^gu <= 16 leaf chunks; writes the multipole
+//           store for levels >= lu and, for each level l < lu, the subtree
+//           root's contribution P(lu-l, .) w_root to its level-l ancestor.
+//   k_m2l   M2L (step 3) of every block of every level, streaming the M2L
+//           matrices through LDS with coalesced loads (each matrix read once
+//           for both parities); blocks above lu first sum the k_up
+//           contributions into their multipoles.  HBM-bound.
+//   k_down  per subtree of 2^gd leaf chunks: L2L (step 4) down the ancestor
+//           chain and the subtree, the near-field band (FP64 VALU), leaf
+//           evaluation (step 5), final scaling, one coalesced store of the
+//           output (which is overwritten, not accumulated).
 //
 // Workspace per line: [multipoles: slot_base(L) x 2M][contributions:
-// lu x 2^(lu+2) x 2M][top M2L: slot_base(ld) x 2M].
+// lu x 2^(lu+2) x 2M][M2L results: slot_base(L) x 2M]; chunk slot
+// slot_base(l) + r holds [parity][M].
 //
 // Kernels are templated on the rank M (MT = M compiled in so that the
 // M-loops unroll; MT = 0 reads M at run time).
@@ -32,16 +35,18 @@ namespace l2c {
 constexpr double kSqrtPi = 1.77245385090551602729816748334;
 constexpr double k2Pi = 0.63661977236758134307553505349;  // 2/pi
 
+typedef double d4 __attribute__((ext_vector_type(4)));
+
 struct WsLayout {
-  size_t w, part, mt, total;
+  size_t w, part, m2l, total;
 };
 __host__ __device__ inline WsLayout ws_layout(const Geo &g) {
   const size_t M2 = 2 * (size_t)g.M;
   WsLayout l;
   l.w = 0;
   l.part = slot_base(g.L) * M2;
-  l.mt = l.part + (size_t)g.lu * ((size_t)4 << g.lu) * M2;
-  l.total = l.mt + slot_base(g.ld) * M2;
+  l.m2l = l.part + (size_t)g.lu * ((size_t)4 << g.lu) * M2;
+  l.total = l.m2l + slot_base(g.L) * M2;
   return l;
 }
 size_t ws_doubles(const Geo &g) { return ws_layout(g).total; }
@@ -52,110 +57,157 @@ __host__ __device__ __forceinline__ int sk4(int c) { return c + (c >> 2); }
 __host__ __device__ __forceinline__ int sk8(int k) { return k + (k >> 3); }
 
 // ---------------------------------------------------------------------------
-// k_up: grid (2^(lu+2), lines), 256 threads.
+// k_up: grid (2^(lu+2), lines), 256 threads.  G = 2^gu <= 16 leaf chunks.
+//
+// Leaf moments W_par (G x M) = X_par (G x s) T_par (s x M) run on
+// v_mfma_f64_16x16x4f64 (A operand: lane l holds A[l%16][l/16]; B: lane l
+// holds B[l/16][l%16]; D: lane l, register v holds D[l/16 + 4v][l%16]).
+// One wave per (parity, 16-column tile of M).
+constexpr int kUpNT = 256;
+
+__host__ __device__ inline int up_xstride(int s) { return s | 1; }  // odd: conflict-free rows
 size_t up_shared_bytes(const Geo &g) {
-  const size_t G = (size_t)1 << g.gu, M2 = 2 * g.M;
-  const bool tl = 2 * (size_t)g.s * g.M * 8 <= 24 * 1024;
-  return 8 * (2 * G * g.s + (tl ? 2 * (size_t)g.s * g.M : 0) + 2 * (size_t)g.M * g.M + G * M2 +
-              (G / 2 + 1) * M2);
+  const size_t M2 = 2 * g.M;
+  return 8 * (2 * 16 * (size_t)up_xstride(g.s) + 16 * M2 + 9 * M2);
 }
 
 template <int MT>
-__global__ void __launch_bounds__(256) k_up(const double *__restrict__ in, LineMap lm, Geo g,
-                                            int dir, const double *__restrict__ T,
-                                            const double *__restrict__ U,
-                                            const double *__restrict__ P,
-                                            double *__restrict__ wsall, size_t wsstride) {
+__global__ void __launch_bounds__(kUpNT) k_up(const double *__restrict__ in, LineMap lm, Geo g,
+                                              int dir, const double *__restrict__ T,
+                                              const double *__restrict__ U,
+                                              const double *__restrict__ P,
+                                              double *__restrict__ wsall, size_t wsstride) {
   extern __shared__ double sm[];
   const int M = MT > 0 ? MT : g.M;
   const int s = g.s, L = g.L, M2 = 2 * M, MM = M * M;
   const int G = 1 << g.gu;
   const int j = blockIdx.x;
-  const int tid = threadIdx.x, NT = blockDim.x;
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   const double *src = in + line_off(lm, blockIdx.y);
   double *ws = wsall + blockIdx.y * wsstride;
   const WsLayout lay = ws_layout(g);
   double *w = ws + lay.w;
   double *part = ws + lay.part;
-  const bool tl = 2 * s * M * 8 <= 24 * 1024;
-  double *xs = sm;                         // [2][G*s]
-  double *Ts = xs + 2 * G * s;             // [2][s][M] (when it fits)
-  double *Us = Ts + (tl ? 2 * s * M : 0);  // [2][M][M]
-  double *w0 = Us + 2 * MM;                // [G][2][M]
-  double *w1 = w0 + G * M2;                // [G/2+1][2][M]
-  const double *Tp = tl ? Ts : T;
+  const int xst = up_xstride(s);
+  double *xs = sm;                  // [2][16][xst]
+  double *w0 = xs + 2 * 16 * xst;   // [16][2][M]
+  double *w1 = w0 + 16 * M2;        // [8+1][2][M]
 
+  // ---- loads, all issued before any use: input, this thread's rows of the
+  // upward operators (k = tid % M) and of its composite operator P
+  const int nin = G * 2 * s;
   const size_t base = (size_t)j * G * 2 * s;
-  for (int e = tid; e < G * 2 * s; e += NT) {
+  double vin[8];
+#pragma unroll
+  for (int q = 0; q < 8; q++) {
+    const int e = q * kUpNT + tid;
     const size_t i = base + e;
-    double v = 0;
-    if (i < (size_t)g.N) {
-      v = src[i * lm.es];
-      if (dir == C2L && i > 0) v *= (double)i;  // v_i = i u_i (src/C/leg2cheb.c:1325-1335)
-    }
-    xs[(e & 1) * G * s + (e >> 1)] = v;
+    vin[q] = (e < nin && i < (size_t)g.N) ? src[i * lm.es] : 0.0;
   }
-  if (tl)
-    for (int e = tid; e < 2 * s * M; e += NT) Ts[e] = T[e];
-  for (int e = tid; e < 2 * MM; e += NT) Us[e] = U[e];
-  __syncthreads();
-
-  // step 1: w_c[k] = sum_m T_par[m][k] x_par[c s + m], four chunks per task
-  const int CPT = G >= 4 ? 4 : G;
-  const int nq = G / CPT;
-  for (int task = tid; task < 2 * nq * M; task += NT) {
-    const int k = task % M, q = (task / M) % nq, sg = task / (M * nq);
-    const double *t = Tp + sg * s * M + k;
-    const double *x = xs + sg * G * s + q * CPT * s;
-    const size_t wl = (slot_base(L - 1) + (size_t)j * G + q * CPT) * M2 + sg * M + k;
-    double *wq = w0 + (q * CPT * 2 + sg) * M + k;
-    if (CPT == 4) {
-      double acc0 = 0, acc1 = 0, acc2 = 0, acc3 = 0;
-      for (int m = 0; m < s; m++) {
-        const double tv = t[m * M];
-        acc0 += tv * x[m];
-        acc1 += tv * x[s + m];
-        acc2 += tv * x[2 * s + m];
-        acc3 += tv * x[3 * s + m];
-      }
-      wq[0] = acc0;
-      wq[M2] = acc1;
-      wq[2 * M2] = acc2;
-      wq[3 * M2] = acc3;
-      w[wl] = acc0;
-      w[wl + M2] = acc1;
-      w[wl + 2 * M2] = acc2;
-      w[wl + 3 * M2] = acc3;
-    } else {
-      double acc0 = 0, acc1 = 0;
-      for (int m = 0; m < s; m++) {
-        const double tv = t[m * M];
-        acc0 += tv * x[m];
-        if (CPT > 1) acc1 += tv * x[s + m];
-      }
-      wq[0] = acc0;
-      w[wl] = acc0;
-      if (CPT > 1) {
-        wq[M2] = acc1;
-        w[wl + M2] = acc1;
-      }
+  const int ku = tid % M;
+  double u0r[MT > 0 ? MT : 1], u1r[MT > 0 ? MT : 1];
+  const int np = g.lu * M2;
+  double pr[MT > 0 ? MT : 1];
+  if (MT > 0) {
+#pragma unroll
+    for (int l = 0; l < (MT > 0 ? MT : 1); l++) {
+      u0r[l] = U[ku * M + l];
+      u1r[l] = U[MM + ku * M + l];
     }
+    if (tid < np) {
+      const int lev = tid / M2, i = tid % M;
+      const int k = g.lu - lev;
+      const size_t d = (size_t)j & (((size_t)1 << k) - 1);
+      const double *prow = P + ((((size_t)1 << k) - 2) + d) * MM + (size_t)i * M;
+#pragma unroll
+      for (int l = 0; l < (MT > 0 ? MT : 1); l++) pr[l] = prow[l];
+    }
+  }
+  for (int e0 = 8 * kUpNT; e0 < nin; e0 += kUpNT) {  // rare: G*2s > 2048
+    const int e = e0 + tid;
+    if (e < nin) {
+      const size_t i = base + e;
+      const double v = i < (size_t)g.N ? src[i * lm.es] : 0.0;
+      xs[(e & 1) * 16 * xst + (e >> 1) / s * xst + (e >> 1) % s] =
+          (dir == C2L && i > 0) ? v * (double)i : v;
+    }
+  }
+#pragma unroll
+  for (int q = 0; q < 8; q++) {
+    const int e = q * kUpNT + tid;
+    if (e < nin) {
+      const size_t i = base + e;
+      const int h = e >> 1;  // half index inside the subtree
+      // v_i = i u_i for C2L (src/C/leg2cheb.c:1325-1335)
+      xs[(e & 1) * 16 * xst + (h / s) * xst + (h % s)] =
+          (dir == C2L && i > 0) ? vin[q] * (double)i : vin[q];
+    }
+  }
+  // rows G..15 of the MFMA A operand are zero
+  for (int e = tid; e < 2 * (16 - G) * xst; e += kUpNT) {
+    const int par = e / ((16 - G) * xst), rem = e % ((16 - G) * xst);
+    xs[par * 16 * xst + G * xst + rem] = 0.0;
   }
   __syncthreads();
 
-  // step 2: parent = U0 child0 + U1 child1 inside the subtree
+  // ---- step 1 on MFMA: W_par[i][k] = sum_m X_par[i][m] T_par[m][k]
+  const int ntile = (M + 15) / 16;
+  for (int task = wave; task < 2 * ntile; task += kUpNT / 64) {
+    const int sg = task / ntile, nt = task % ntile;
+    const int ia = lane & 15, ka = lane >> 4;  // A: row i, k-offset
+    const int kb = nt * 16 + (lane & 15);      // B: column k
+    const double *xrow = xs + sg * 16 * xst + ia * xst;
+    const double *tcol = T + (size_t)sg * s * M + kb;
+    d4 acc = {0, 0, 0, 0};
+    for (int m0 = 0; m0 < s; m0 += 4) {
+      const int m = m0 + ka;
+      const double a = m < s ? xrow[m] : 0.0;
+      const double b = (m < s && kb < M) ? tcol[(size_t)m * M] : 0.0;
+      acc = __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, acc, 0, 0, 0);
+    }
+#pragma unroll
+    for (int v = 0; v < 4; v++) {
+      const int i = (lane >> 4) + 4 * v;
+      if (i < G && kb < M) {
+        w0[(i * 2 + sg) * M + kb] = acc[v];
+        w[(slot_base(L - 1) + (size_t)j * G + i) * M2 + sg * M + kb] = acc[v];
+      }
+    }
+  }
+  __syncthreads();
+
+  // ---- step 2: parent = U0 child0 + U1 child1 inside the subtree; thread
+  // (k = tid % M) keeps its operator rows in registers
+  const int npair = kUpNT / M;  // (chunk, parity) pairs handled per sweep
   int n = G;
   for (int lev = L - 2; lev >= g.lu; lev--) {
     n >>= 1;
-    for (int o = tid; o < n * M2; o += NT) {
-      const int c = o / M2, sg = (o / M) & 1, k = o % M;
-      const double *x0 = w0 + (2 * c * 2 + sg) * M;
-      const double *x1 = w0 + ((2 * c + 1) * 2 + sg) * M;
-      const double *u0 = Us + k * M, *u1 = Us + MM + k * M;
-      double acc = 0;
-      for (int l = 0; l < M; l++) acc += u0[l] * x0[l] + u1[l] * x1[l];
-      w1[o] = acc;
-      w[(slot_base(lev) + (size_t)j * n + c) * M2 + (o % M2)] = acc;
+    if (tid < npair * M) {
+      for (int cs = tid / M; cs < 2 * n; cs += npair) {
+        const int c = cs >> 1, sg = cs & 1;
+        const double *x0 = w0 + (2 * c * 2 + sg) * M;
+        const double *x1 = w0 + ((2 * c + 1) * 2 + sg) * M;
+        double a0 = 0, a1 = 0, a2 = 0, a3 = 0;
+        if (MT > 0) {
+#pragma unroll
+          for (int l = 0; l < (MT > 0 ? MT : 1); l += 2) {
+            a0 += u0r[l] * x0[l];
+            a1 += u1r[l] * x1[l];
+            if (l + 1 < M) {
+              a2 += u0r[l + 1] * x0[l + 1];
+              a3 += u1r[l + 1] * x1[l + 1];
+            }
+          }
+        } else {
+          for (int l = 0; l < M; l++) {
+            a0 += U[ku * M + l] * x0[l];
+            a1 += U[MM + ku * M + l] * x1[l];
+          }
+        }
+        const double acc = (a0 + a2) + (a1 + a3);
+        w1[(c * 2 + sg) * M + ku] = acc;
+        w[(slot_base(lev) + (size_t)j * n + c) * M2 + sg * M + ku] = acc;
+      }
     }
     __syncthreads();
     double *tmp = w0;
@@ -163,116 +215,202 @@ __global__ void __launch_bounds__(256) k_up(const double *__restrict__ in, LineM
     w1 = tmp;
   }
 
-  // contributions of the root (level lu, chunk j) to its ancestors:
+  // ---- contributions of the root (level lu, chunk j) to its ancestors:
   // part[l][j] = P(lu-l, j mod 2^(lu-l)) w_root
-  for (int task = tid; task < g.lu * M2; task += NT) {
+  for (int task = tid; task < np; task += kUpNT) {
     const int lev = task / M2, o = task % M2, sg = o / M, i = o % M;
-    const int k = g.lu - lev;
-    const size_t d = (size_t)j & (((size_t)1 << k) - 1);
-    const double *pr = P + ((((size_t)1 << k) - 2) + d) * MM + (size_t)i * M;
     const double *x = w0 + sg * M;
     double acc = 0;
-    for (int l = 0; l < M; l++) acc += pr[l] * x[l];
+    if (MT > 0 && task == tid) {
+#pragma unroll
+      for (int l = 0; l < (MT > 0 ? MT : 1); l++) acc += pr[l] * x[l];
+    } else {
+      const int k = g.lu - lev;
+      const size_t d = (size_t)j & (((size_t)1 << k) - 1);
+      const double *prow = P + ((((size_t)1 << k) - 2) + d) * MM + (size_t)i * M;
+      for (int l = 0; l < M; l++) acc += prow[l] * x[l];
+    }
     part[((size_t)lev * ((size_t)4 << g.lu) + j) * M2 + o] = acc;
   }
 }
 
 // ---------------------------------------------------------------------------
-// M2L of row chunk r at `level` for both parities (src/C/leg2cheb.c:1419-1435):
-// block b = r/2; r even: A(b,0) w[r+2] + A(b,1) w[r+3]; r odd: A(b,2) w[r+2].
-// w0, w1 point at the [2][M] multipoles of chunks r+2 and r+3.
-template <int MT>
-__device__ __forceinline__ void m2l_rowk(const double *__restrict__ A, int level, size_t r,
-                                         int k, int Mr, const double *w0, const double *w1,
-                                         double &out0, double &out1) {
-  const int M = MT > 0 ? MT : Mr;
-  const size_t MM = (size_t)M * M;
-  const double *Ab = A + (3 * total_blocks(level) + 3 * (r >> 1)) * MM;
-  if ((r & 1) == 0) {
-    const double *a0 = Ab + (size_t)k * M, *a1 = Ab + MM + (size_t)k * M;
-    double s00 = 0, s01 = 0, s10 = 0, s11 = 0;
+// k_m2l: M2L of every block (src/C/leg2cheb.c:1419-1435).  Block b of level l
+// holds A(b,0), A(b,1), A(b,2); row chunk 2b gets A(b,0) w[2b+2] +
+// A(b,1) w[2b+3], row chunk 2b+1 gets A(b,2) w[2b+3]; both parities at once.
+// grid: total_blocks(lu) work-groups for the blocks above lu (one each, with
+// the multipoles summed from the k_up contributions in a fixed order), then
+// one work-group per four blocks below (one block per wave).  256 threads.
+constexpr int kM2lNT = 256;
+// per wave: three padded matrices, two multipoles, 3M row results
+__host__ __device__ inline int m2l_wave_lds(int M) { return 3 * M * (M + 1) + 4 * M + 6 * M; }
+// Blocks per work-group below lu: four (one per wave) when four staged blocks
+// fit in 64 KiB of LDS (M <= 22), else one block per work-group.
+int m2l_bpw(const Geo &g) {
+  return 8 * (size_t)(4 * m2l_wave_lds(g.M) + kM2lNT) <= 64 * 1024 ? 4 : 1;
+}
+size_t m2l_shared_bytes(const Geo &g) {
+  return 8 * (size_t)(m2l_bpw(g) * m2l_wave_lds(g.M) + kM2lNT);
+}
+
+// Order LDS traffic between lanes of one wave (the wave's DS operations run
+// in order; the fences stop the compiler from moving them across).
+__device__ __forceinline__ void wave_sync() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+// Evaluate one staged block with the 64 lanes of a wave (WG = false) or the
+// whole work-group (WG = true).  Al: [3][M][M+1] (padded rows),
+// wl: [chunk 2b+2, 2b+3][parity][M], res: [3M][2] scratch.
+template <int MT, bool WG>
+__device__ __forceinline__ void m2l_block_compute(const double *Al, const double *wl,
+                                                  double *res, int M, int lane,
+                                                  double *out_even, double *out_odd) {
+  constexpr int step = WG ? kM2lNT : 64;
+  for (int row = lane; row < 3 * M; row += step) {
+    const int sub = row / M, k = row % M;
+    const double *arow = Al + row * (M + 1);
+    const double *wv = wl + (sub == 0 ? 0 : 2 * M);
+    double o0 = 0, o1 = 0;
 #pragma unroll
-    for (int l = 0; l < M; l++) {
-      const double x0 = a0[l], x1 = a1[l];
-      s00 += x0 * w0[l];
-      s01 += x0 * w0[M + l];
-      s10 += x1 * w1[l];
-      s11 += x1 * w1[M + l];
+    for (int l = 0; l < (MT > 0 ? MT : 64); l++) {
+      if (MT == 0 && l >= M) break;
+      const double a = arow[l];
+      o0 += a * wv[l];
+      o1 += a * wv[M + l];
     }
-    out0 = s00 + s10;
-    out1 = s01 + s11;
-  } else {
-    const double *a2 = Ab + 2 * MM + (size_t)k * M;
-    double s0 = 0, s1 = 0;
-#pragma unroll
-    for (int l = 0; l < M; l++) {
-      const double x = a2[l];
-      s0 += x * w0[l];
-      s1 += x * w0[M + l];
-    }
-    out0 = s0;
-    out1 = s1;
+    (void)k;
+    res[2 * row] = o0;
+    res[2 * row + 1] = o1;
+  }
+  if (WG)
+    __syncthreads();
+  else
+    wave_sync();
+  for (int k = lane; k < M; k += step) {
+    // row chunk 2b: (A(b,0) w[2b+2]) + (A(b,1) w[2b+3]); row chunk 2b+1: A(b,2) w[2b+3]
+    out_even[k] = res[2 * k] + res[2 * (M + k)];
+    out_even[M + k] = res[2 * k + 1] + res[2 * (M + k) + 1];
+    out_odd[k] = res[2 * (2 * M + k)];
+    out_odd[M + k] = res[2 * (2 * M + k) + 1];
   }
 }
 
-// ---------------------------------------------------------------------------
-// k_top: grid (slot_base(ld), lines), 256 threads.  M2L of every row chunk
-// above level ld; multipoles above lu are summed from the k_up contributions
-// in a fixed order.
 template <int MT>
-__global__ void __launch_bounds__(256) k_top(Geo g, const double *__restrict__ A,
-                                             double *__restrict__ wsall, size_t wsstride) {
-  __shared__ double wv[2 * 96];
-  __shared__ double red[256];
-  const int M = MT > 0 ? MT : g.M, M2 = 2 * M;
-  const int tid = threadIdx.x, NT = blockDim.x;
+__global__ void __launch_bounds__(kM2lNT) k_m2l(Geo g, const double *__restrict__ A,
+                                                double *__restrict__ wsall, size_t wsstride,
+                                                int bpw) {
+  extern __shared__ double sm[];
+  const int M = MT > 0 ? MT : g.M, M2 = 2 * M, MM = M * M, Mp = M + 1;
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   double *ws = wsall + blockIdx.y * wsstride;
   const WsLayout lay = ws_layout(g);
-  const int q = blockIdx.x;
-  int lev = 0;
-  while ((int)slot_base(lev + 1) <= q) lev++;
-  const size_t r = (size_t)(q - (int)slot_base(lev));
-  double *mt = ws + lay.mt + (size_t)q * M2;
-  if (r >= ((size_t)4 << lev) - 2) {
-    for (int o = tid; o < M2; o += NT) mt[o] = 0.0;
+  const double *w = ws + lay.w;
+  double *mo = ws + lay.m2l;
+  const int ntop = (int)total_blocks(g.lu);
+  const int ntot = (int)total_blocks(g.L);
+  const int WL = m2l_wave_lds(M);
+
+  if ((int)blockIdx.x < ntop || bpw == 1) {
+    // ---- one block per work-group: every block above lu (multipoles summed
+    // from the k_up contributions), and every block when bpw == 1
+    const int B = blockIdx.x;
+    if (B >= ntot) return;
+    int lev = 0;
+    while ((int)total_blocks(lev + 1) <= B) lev++;
+    const int b = B - (int)total_blocks(lev);
+    double *Al = sm, *wl = sm + 3 * M * Mp, *red = sm + WL;
+    const double *Ab = A + (size_t)(3 * total_blocks(lev) + 3 * b) * MM;
+    double va[8];
+#pragma unroll
+    for (int q = 0; q < 8; q++) {
+      const int e = q * kM2lNT + tid;
+      va[q] = e < 3 * MM ? Ab[e] : 0.0;
+    }
+    const int nw = 2 * M2;  // multipole doubles of chunks 2b+2, 2b+3
+    if (lev >= g.lu) {
+      const double *wb = w + (slot_base(lev) + 2 * (size_t)b + 2) * M2;
+      for (int o = tid; o < nw; o += kM2lNT) wl[o] = wb[o];
+    } else {
+      const size_t nd = (size_t)1 << (g.lu - lev);
+      const int ngrp = std::max(1, kM2lNT / nw);
+      for (int o0 = 0; o0 < nw; o0 += kM2lNT) {
+        const int o = o0 + tid % std::min(nw, kM2lNT), grp = tid / std::min(nw, kM2lNT);
+        double acc = 0;
+        if (grp < ngrp && o < nw) {
+          const int ci = o / M2, oo = o % M2;
+          const double *pp = ws + lay.part +
+                             ((size_t)lev * ((size_t)4 << g.lu) + (2 * (size_t)b + 2 + ci) * nd) * M2 + oo;
+          for (size_t d0 = grp; d0 < nd; d0 += 8 * (size_t)ngrp) {
+            double v[8];
+#pragma unroll
+            for (int q = 0; q < 8; q++) {
+              const size_t d = d0 + (size_t)q * ngrp;
+              v[q] = d < nd ? pp[d * M2] : 0.0;
+            }
+#pragma unroll
+            for (int q = 0; q < 8; q++) acc += v[q];
+          }
+        }
+        red[tid] = acc;
+        __syncthreads();
+        if (tid < std::min(nw, kM2lNT) && o0 + tid < nw) {
+          double sum = 0;
+          for (int gi = 0; gi < ngrp; gi++) sum += red[gi * std::min(nw, kM2lNT) + tid];
+          wl[o0 + tid] = sum;
+        }
+        __syncthreads();
+      }
+    }
+#pragma unroll
+    for (int q = 0; q < 8; q++) {
+      const int e = q * kM2lNT + tid;
+      if (e < 3 * MM) Al[(e / M) * Mp + e % M] = va[q];
+    }
+    for (int e = 8 * kM2lNT + tid; e < 3 * MM; e += kM2lNT) Al[(e / M) * Mp + e % M] = Ab[e];
+    __syncthreads();
+    m2l_block_compute<MT, true>(Al, wl, wl + nw, M, tid,
+                                mo + (slot_base(lev) + 2 * (size_t)b) * M2,
+                                mo + (slot_base(lev) + 2 * (size_t)b + 1) * M2);
     return;
   }
-  const int nc = (r & 1) ? 1 : 2;
-  if (lev >= g.lu) {
-    const double *w = ws + lay.w + (slot_base(lev) + r + 2) * M2;
-    for (int o = tid; o < nc * M2; o += NT) wv[o] = w[o];
-  } else {
-    const int k = g.lu - lev;
-    const size_t nd = (size_t)1 << k;
-    const int ngrp = NT / M2;
-    const int o = tid % M2, grp = tid / M2;
-    for (int ci = 0; ci < nc; ci++) {
-      double acc = 0;
-      if (grp < ngrp) {
-        const double *pp = ws + lay.part +
-                           ((size_t)lev * ((size_t)4 << g.lu) + (r + 2 + ci) * nd) * M2 + o;
-        for (size_t d = grp; d < nd; d += ngrp) acc += pp[d * M2];
-      }
-      red[tid] = acc;
-      __syncthreads();
-      if (tid < M2) {
-        double sum = 0;
-        for (int gi = 0; gi < ngrp; gi++) sum += red[gi * M2 + tid];
-        wv[ci * M2 + tid] = sum;
-      }
-      __syncthreads();
-    }
+
+  // ---- blocks at levels >= lu: one per wave
+  const int B = ntop + ((int)blockIdx.x - ntop) * 4 + wave;
+  if (B >= ntot) return;  // no work-group barrier below this point
+  int lev = g.lu;
+  while ((int)total_blocks(lev + 1) <= B) lev++;
+  const int b = B - (int)total_blocks(lev);
+  double *Al = sm + wave * WL, *wl = Al + 3 * M * Mp;
+  const double *Ab = A + (size_t)(3 * total_blocks(lev) + 3 * b) * MM;
+  const double *wb = w + (slot_base(lev) + 2 * (size_t)b + 2) * M2;  // chunks 2b+2, 2b+3
+  constexpr int QA = MT > 0 ? (3 * MT * MT + 63) / 64 : 16;
+  double va[QA];
+#pragma unroll
+  for (int q = 0; q < QA; q++) {
+    const int e = q * 64 + lane;
+    va[q] = e < 3 * MM ? Ab[e] : 0.0;
   }
-  __syncthreads();
-  for (int k = tid; k < M; k += NT) {
-    double o0, o1;
-    m2l_rowk<MT>(A, lev, r, k, M, wv, wv + M2, o0, o1);
-    mt[k] = o0;
-    mt[M + k] = o1;
+  const double w0v = lane < 2 * M2 ? wb[lane] : 0.0;  // 2*M2 <= 192
+  const double w1v = lane + 64 < 2 * M2 ? wb[lane + 64] : 0.0;
+  const double w2v = lane + 128 < 2 * M2 ? wb[lane + 128] : 0.0;
+#pragma unroll
+  for (int q = 0; q < QA; q++) {
+    const int e = q * 64 + lane;
+    if (e < 3 * MM) Al[(e / M) * Mp + e % M] = va[q];
   }
+  for (int e = QA * 64 + lane; e < 3 * MM; e += 64) Al[(e / M) * Mp + e % M] = Ab[e];
+  if (lane < 2 * M2) wl[lane] = w0v;
+  if (lane + 64 < 2 * M2) wl[lane + 64] = w1v;
+  if (lane + 128 < 2 * M2) wl[lane + 128] = w2v;
+  wave_sync();
+  m2l_block_compute<MT, false>(Al, wl, wl + 2 * M2, M, lane,
+                               mo + (slot_base(lev) + 2 * (size_t)b) * M2,
+                               mo + (slot_base(lev) + 2 * (size_t)b + 1) * M2);
 }
 
-// ---------------------------------------------------------------------------
 // Near-field band for four consecutive half-rows r0..r0+3 of one parity:
 // columns c = r0 + t, t < len (len equal for the four rows), term
 //   L2C: (a[t-rho] a[r+c+par]) x[c]            (src/C/leg2cheb.c:259,263)
@@ -364,13 +502,16 @@ __device__ __forceinline__ void band_dispatch(int sg, int aligned, const double 
   }
 }
 
+
 // ---------------------------------------------------------------------------
 // k_down: grid (2^(ld+2), lines), 128 threads (2 waves).
 constexpr int kR = 4;  // band rows per register block
 constexpr int kDownNT = 128;
 
+// LDS plan of k_down (offsets in doubles).  The M2L vectors and the chain
+// buffer are dead once the band phase starts; the output staging reuses them.
 struct DownLds {
-  int xs, aw, hd, D, cres, cb, wsub, mch, out, total;  // offsets in doubles
+  int xs, aw, hd, cres, mres, cb, out, total;
 };
 __host__ __device__ inline DownLds down_lds(const Geo &g) {
   const int G = 1 << g.gd, s = g.s, M2 = 2 * g.M;
@@ -382,29 +523,34 @@ __host__ __device__ inline DownLds down_lds(const Geo &g) {
   d.xs = 0;
   d.aw = d.xs + 2 * nxs;
   d.hd = d.aw + nas;
-  d.D = d.hd + 2 * s + 16;
-  d.cres = d.D + 2 * g.M * g.M;
-  d.cb = d.cres + (2 * G - 1) * M2;
-  d.wsub = d.cb + 2 * M2;
-  d.mch = d.wsub + (2 * G - 1 + 2 * (g.gd + 1)) * M2;
-  d.out = d.mch + std::max(g.ld, 1) * M2;
-  d.total = d.out + 2 * G * s;
+  d.cres = d.hd + 2 * s + 16;
+  d.mres = d.cres + (2 * G - 1) * M2;
+  d.cb = d.mres + (g.ld + 2 * G - 1) * M2;
+  d.out = d.mres;
+  d.total = std::max(d.cb + 2 * M2, d.out + 2 * G * s);
   return d;
 }
 size_t down_shared_bytes(const Geo &g) { return 8 * (size_t)down_lds(g).total; }
 
+// Subtree chunk slot cl (level ld + lam, position ci) of work-group Rr.
+__device__ __forceinline__ void sub_slot(int cl, int &lam, int &ci) {
+  lam = 0;
+  while ((2 << lam) - 1 <= cl) lam++;
+  ci = cl - ((1 << lam) - 1);
+}
+
 template <int DIR, int PRE, int MT>
 __global__ void __launch_bounds__(kDownNT) k_down(
     const double *__restrict__ in, double *__restrict__ out, LineMap lm, Geo g, int band_only,
-    const double *__restrict__ A, double *__restrict__ wsall, size_t wsstride,
-    const double *__restrict__ Dg, const double *__restrict__ Tg, const double *__restrict__ tab,
+    const double *__restrict__ wsall, size_t wsstride, const double *__restrict__ Dg,
+    const double *__restrict__ Tg, const double *__restrict__ tab,
     const double *__restrict__ htab) {
   extern __shared__ double sm[];
   const int M = MT > 0 ? MT : g.M;
   const int s = g.s, L = g.L, N = g.N, M2 = 2 * M, MM = M * M;
   const int G = 1 << g.gd;
   const int Rr = blockIdx.x;
-  const int tid = threadIdx.x, NT = blockDim.x;
+  const int tid = threadIdx.x, NT = kDownNT;
   const size_t row0 = (size_t)Rr * G * 2 * s;
   if (row0 >= (size_t)N) return;
   const int nr = (int)std::min((size_t)G * 2 * s, (size_t)N - row0);
@@ -413,151 +559,189 @@ __global__ void __launch_bounds__(kDownNT) k_down(
   double *dst = out + off;
   const double *ws = wsall + blockIdx.y * wsstride;
   const WsLayout lay = ws_layout(g);
+  const double *m2l = ws + lay.m2l;
   const DownLds ld = down_lds(g);
   double *xs = sm + ld.xs;
   const int nxs = (ld.aw - ld.xs) / 2;
   double *aw = sm + ld.aw;
   double *hd = sm + ld.hd;
-  double *Ds = sm + ld.D;
   double *cres = sm + ld.cres;
+  double *mres = sm + ld.mres;
   double *cb = sm + ld.cb;
-  double *wsub = sm + ld.wsub;
-  double *mch = sm + ld.mch;
   double *ob = sm + ld.out;
 
-  // ---- phase 0: windows (input split by parity, Lambda, band head), the
-  // subtree's multipoles and the ancestor chain's M2L vectors
+  // ---- phase 0: every load issued before any store
   const int nwin = G * 2 * s + 2 * s + 32;
-  for (int e = tid; e < nwin; e += NT) {
+  constexpr int QW = 10;
+  double vx[QW], va[QW];
+#pragma unroll
+  for (int q = 0; q < QW; q++) {
+    const int e = q * NT + tid;
     const size_t i = row0 + e;
-    double v = 0;
-    if (i < (size_t)N) {
-      v = src[i * lm.es];
-      if (DIR == 1 && i > 0) v *= (double)i;  // v_i = i u_i
+    const bool ok = e < nwin && i < (size_t)N;
+    vx[q] = ok ? src[i * lm.es] : 0.0;
+    va[q] = ok ? tab[i] : 0.0;
+  }
+  const int nh = std::min(2 * s, DIR == 0 ? N : (N + 1) / 2);
+  const int nhd = 2 * s + 16;
+  double vh[3];
+#pragma unroll
+  for (int q = 0; q < 3; q++) {
+    const int e = q * NT + tid;
+    vh[q] = e < nh ? htab[e] : 0.0;
+  }
+  // M2L vectors: the ancestor chain (levels 0..ld-1) then the subtree slots
+  const int nm = band_only ? 0 : (g.ld + 2 * G - 1) * M2;
+  constexpr int QM = 8;
+  double vm[QM];
+#pragma unroll
+  for (int q = 0; q < QM; q++) {
+    const int e = q * NT + tid;
+    double v = 0.0;
+    if (e < nm) {
+      const int slot = e / M2, o = e % M2;
+      int lev;
+      size_t r;
+      if (slot < g.ld) {
+        lev = slot;
+        r = (size_t)Rr >> (g.ld - lev);
+      } else {
+        int lam, ci;
+        sub_slot(slot - g.ld, lam, ci);
+        lev = g.ld + lam;
+        r = ((size_t)Rr << lam) + ci;
+      }
+      if (r < ((size_t)4 << lev) - 2) v = m2l[(slot_base(lev) + r) * M2 + o];
     }
+    vm[q] = v;
+  }
+  // this thread's rows of the two L2L operators (k = tid % M)
+  const int kd = tid % M;
+  double d0r[MT > 0 ? MT : 1], d1r[MT > 0 ? MT : 1];
+  if (MT > 0 && !band_only) {
+#pragma unroll
+    for (int l = 0; l < (MT > 0 ? MT : 1); l++) {
+      d0r[l] = Dg[kd * M + l];
+      d1r[l] = Dg[MM + kd * M + l];
+    }
+  }
+#pragma unroll
+  for (int q = 0; q < QW; q++) {
+    const int e = q * NT + tid;
+    if (e < nwin) {
+      const size_t i = row0 + e;
+      xs[(e & 1) * nxs + sk4(e >> 1)] = (DIR == 1 && i > 0) ? vx[q] * (double)i : vx[q];  // v_i = i u_i
+      aw[sk8(e)] = va[q];
+    }
+  }
+  for (int e = QW * NT + tid; e < nwin; e += NT) {
+    const size_t i = row0 + e;
+    double v = i < (size_t)N ? src[i * lm.es] : 0.0;
+    if (DIR == 1 && i > 0) v *= (double)i;
     xs[(e & 1) * nxs + sk4(e >> 1)] = v;
     aw[sk8(e)] = i < (size_t)N ? tab[i] : 0.0;
   }
-  const int nh = std::min(2 * s, DIR == 0 ? N : (N + 1) / 2);
-  for (int e = tid; e < 2 * s + 16; e += NT) hd[e] = e < nh ? htab[e] : 0.0;
-  if (!band_only) {
-    for (int e = tid; e < 2 * MM; e += NT) Ds[e] = Dg[e];
-    const double *w = ws + lay.w;
-    int sbase = 0;
-    for (int lam = 0; lam <= g.gd; lam++) {
-      const int lev = g.ld + lam, n = (1 << lam) + 2;
-      const size_t c0 = ((size_t)Rr << lam) + 2, cmax = (size_t)4 << lev;
-      for (int e = tid; e < n * M2; e += NT) {
-        const size_t c = c0 + e / M2;
-        wsub[sbase * M2 + e] = c < cmax ? w[(slot_base(lev) + c) * M2 + e % M2] : 0.0;
-      }
-      sbase += n;
-    }
-    const double *mt = ws + lay.mt;
-    for (int e = tid; e < g.ld * M2; e += NT) {
-      const int lev = e / M2;
-      const size_t r = (size_t)Rr >> (g.ld - lev);
-      mch[e] = mt[(slot_base(lev) + r) * M2 + e % M2];
-    }
+#pragma unroll
+  for (int q = 0; q < 3; q++) {
+    const int e = q * NT + tid;
+    if (e < nhd) hd[e] = vh[q];
   }
+  for (int e = 3 * NT + tid; e < nhd; e += NT) hd[e] = e < nh ? htab[e] : 0.0;
+#pragma unroll
+  for (int q = 0; q < QM; q++) {
+    const int e = q * NT + tid;
+    if (e < nm) mres[e] = vm[q];
+  }
+  for (int e = QM * NT + tid; e < nm; e += NT) {
+    const int slot = e / M2, o = e % M2;
+    int lev;
+    size_t r;
+    if (slot < g.ld) {
+      lev = slot;
+      r = (size_t)Rr >> (g.ld - lev);
+    } else {
+      int lam, ci;
+      sub_slot(slot - g.ld, lam, ci);
+      lev = g.ld + lam;
+      r = ((size_t)Rr << lam) + ci;
+    }
+    mres[e] = r < ((size_t)4 << lev) - 2 ? m2l[(slot_base(lev) + r) * M2 + o] : 0.0;
+  }
+  if (!band_only && tid < M2) cb[tid] = 0.0;
   __syncthreads();
 
-  // ---- phase 1: M2L of the subtree levels ld .. L-1 into cres
-  if (!band_only) {
-    const int ntask = (2 * G - 1) * M;
-    for (int task = tid; task < ntask; task += NT) {
-      const int k = task % M, cl = task / M;  // cl: chunk slot in the subtree
-      int lam = 0;
-      while ((2 << lam) - 1 <= cl) lam++;
-      const int ci = cl - ((1 << lam) - 1);
-      const int lev = g.ld + lam;
-      const size_t r = ((size_t)Rr << lam) + ci;
-      double o0 = 0, o1 = 0;
-      if (r < ((size_t)4 << lev) - 2) {
-        const double *wb = wsub + ((1 << lam) - 1 + 2 * lam + ci) * M2;
-        m2l_rowk<MT>(A, lev, r, k, M, wb, wb + M2, o0, o1);
+  // ---- phase 2: L2L (src/C/leg2cheb.c:1437-1449) down the ancestor chain,
+  // the root and the subtree: c = D(p) c_parent + m2l
+  auto l2l = [&](int p, const double *par) -> double {
+    double a0 = 0, a1 = 0;
+    if (MT > 0) {
+#pragma unroll
+      for (int l = 0; l < (MT > 0 ? MT : 1); l += 2) {
+        a0 += (p ? d1r[l] : d0r[l]) * par[l];
+        if (l + 1 < M) a1 += (p ? d1r[l + 1] : d0r[l + 1]) * par[l + 1];
       }
-      cres[cl * M2 + k] = o0;
-      cres[cl * M2 + M + k] = o1;
+    } else {
+      for (int l = 0; l < M; l++) a0 += Dg[p * MM + kd * M + l] * par[l];
     }
-  }
-  __syncthreads();
-
-  // ---- phase 2: L2L down the ancestor chain, then inside the subtree
+    return a0 + a1;
+  };
   if (!band_only) {
     double *cur = cb, *nxt = cb + M2;
-    for (int o = tid; o < M2; o += NT) cur[o] = 0.0;
-    __syncthreads();
+    const int sg0 = tid / M;  // valid for tid < M2
     for (int lev = 0; lev < g.ld; lev++) {
       const size_t r = (size_t)Rr >> (g.ld - lev);
       const bool valid = r < ((size_t)4 << lev) - 2;
-      const int p = (int)(r & 1);
-      for (int o = tid; o < M2; o += NT) {
-        const int sg = o / M, k = o % M;
-        double v = 0;
-        if (valid) {
-          const double *dr = Ds + p * MM + k * M;
-          const double *par = cur + sg * M;
-#pragma unroll
-          for (int jj = 0; jj < M; jj++) v += dr[jj] * par[jj];
-          v += mch[lev * M2 + o];
-        }
-        nxt[o] = v;
-      }
+      double v = 0;
+      if (tid < M2 && valid) v = l2l((int)(r & 1), cur + sg0 * M) + mres[lev * M2 + tid];
+      if (tid < M2) nxt[tid] = v;
       __syncthreads();
       double *t = cur;
       cur = nxt;
       nxt = t;
     }
-    // root chunk (level ld)
-    if (g.ld > 0 && (size_t)Rr < ((size_t)4 << g.ld) - 2) {
-      const int p = Rr & 1;
-      for (int o = tid; o < M2; o += NT) {
-        const int sg = o / M, k = o % M;
-        const double *dr = Ds + p * MM + k * M;
-        const double *par = cur + sg * M;
-        double v = 0;
-#pragma unroll
-        for (int jj = 0; jj < M; jj++) v += dr[jj] * par[jj];
-        cres[o] += v;
-      }
+    if (tid < M2) {
+      const bool valid = (size_t)Rr < ((size_t)4 << g.ld) - 2;
+      double v = 0;
+      if (valid) v = (g.ld > 0 ? l2l(Rr & 1, cur + sg0 * M) : 0.0) + mres[g.ld * M2 + tid];
+      cres[tid] = v;
     }
     __syncthreads();
+    const int ngrp = NT / M2;
     for (int lam = 1; lam <= g.gd; lam++) {
       const int n = 1 << lam, lev = g.ld + lam;
       const int b0 = n - 1, bp = (n >> 1) - 1;
-      for (int o = tid; o < n * M2; o += NT) {
-        const int ci = o / M2, oo = o % M2, sg = oo / M, k = oo % M;
-        const size_t r = ((size_t)Rr << lam) + ci;
-        if (r < ((size_t)4 << lev) - 2) {
-          const double *dr = Ds + (int)(r & 1) * MM + k * M;
-          const double *par = cres + (bp + (ci >> 1)) * M2 + sg * M;
+      if (tid < ngrp * M2) {
+        const int oo = tid % M2, sg = oo / M;
+        for (int ci = tid / M2; ci < n; ci += ngrp) {
+          const size_t r = ((size_t)Rr << lam) + ci;
           double v = 0;
-#pragma unroll
-          for (int jj = 0; jj < M; jj++) v += dr[jj] * par[jj];
-          cres[(b0 + ci) * M2 + oo] += v;
+          if (r < ((size_t)4 << lev) - 2)
+            v = l2l((int)(r & 1), cres + (bp + (ci >> 1)) * M2 + sg * M) +
+                mres[(g.ld + b0 + ci) * M2 + oo];
+          cres[(b0 + ci) * M2 + oo] = v;
         }
       }
       __syncthreads();
     }
   }
 
-  // ---- phase 3: band + leaf evaluation, paired row groups per thread
-  const int ngrp = (s + kR - 1) / kR, npair = (ngrp + 1) / 2;
-  const int ntask = 2 * G * npair;
+  // ---- phase 3: band + leaf evaluation.  Task (chunk, group gam): rows
+  // gam*4.. of parity 0, then rows (ngrp-1-gam)*4.. of parity 1, so every
+  // thread runs the same number of columns, parity is uniform across a wave
+  // in each half, and 16 consecutive lanes take 16 consecutive row groups.
+  const int ngrp = (s + kR - 1) / kR;
+  const int ntask = G * ngrp;
   const int nLm1 = g.nL - 1;
   const size_t nvalid = ((size_t)2 << L) - 2;
   const size_t c0 = (size_t)Rr * G * s;
   const int aligned = (s & 3) == 0;
   for (int task = tid; task < ntask; task += NT) {
-    const int pp = task % npair, gsg = task / npair;
-    const int gch = gsg % G, sg = gsg / G;
+    const int pp = task % ngrp, gch = task / ngrp;
     const size_t beta = (size_t)Rr * G + gch;
-    const int cnt = (N - sg + 1) / 2;  // elements of parity sg
-    const size_t cend = beta < (size_t)nLm1 ? (beta + 2) * s : (size_t)cnt;
-    for (int half = 0; half < 2; half++) {
-      const int gam = half == 0 ? pp : ngrp - 1 - pp;
-      if (half == 1 && gam == pp) break;
+    for (int sg = 0; sg < 2; sg++) {
+      const int gam = sg == 0 ? pp : ngrp - 1 - pp;
+      const int cnt = (N - sg + 1) / 2;  // elements of parity sg
+      const size_t cend = beta < (size_t)nLm1 ? (beta + 2) * s : (size_t)cnt;
       const size_t r0 = beta * s + (size_t)gam * kR;
       if (r0 >= (size_t)cnt) continue;
       const int rl0 = (int)(r0 - c0);
@@ -680,30 +864,29 @@ static void launch_fmm_m(fmm_plan_s *p, const double *d_in, double *d_out, size_
   if (!band_only) {
     dim3 gu((unsigned)((size_t)4 << g.lu), (unsigned)nlines);
     prof_begin(p->prof, K_UP_SUBTREE, st);
-    k_up<MT><<<gu, 256, up_shared_bytes(g), st>>>(d_in, lm, g, (int)dir, p->t.T, p->t.U, p->t.P,
-                                                  ws, wsstride);
+    k_up<MT><<<gu, kUpNT, up_shared_bytes(g), st>>>(d_in, lm, g, (int)dir, p->t.T, p->t.U,
+                                                    p->t.P, ws, wsstride);
     prof_end(p->prof, st);
-    if (g.ld > 0) {
-      dim3 gt((unsigned)slot_base(g.ld), (unsigned)nlines);
-      prof_begin(p->prof, K_UP_TOP, st);
-      k_top<MT><<<gt, 256, 0, st>>>(g, p->t.A[dir], ws, wsstride);
-      prof_end(p->prof, st);
-    }
+    const int ntop = (int)total_blocks(g.lu), ntot = (int)total_blocks(g.L);
+    const int bpw = m2l_bpw(g);
+    dim3 gm((unsigned)(ntop + (ntot - ntop + bpw - 1) / bpw), (unsigned)nlines);
+    prof_begin(p->prof, K_M2L, st);
+    k_m2l<MT><<<gm, kM2lNT, m2l_shared_bytes(g), st>>>(g, p->t.A[dir], ws, wsstride, bpw);
+    prof_end(p->prof, st);
   }
   dim3 gd((unsigned)((size_t)4 << g.ld), (unsigned)nlines);
   const size_t shm = down_shared_bytes(g);
   const double *tab = dir == L2C ? p->t.a : p->t.an;
   const double *htab = dir == L2C ? p->t.a : p->t.dn;
-  const double *A = p->t.A[dir];
   prof_begin(p->prof, K_DOWN_BAND, st);
   if (dir == L2C)
-    k_down<0, 0, MT><<<gd, kDownNT, shm, st>>>(d_in, d_out, lm, g, band_only, A, ws, wsstride,
+    k_down<0, 0, MT><<<gd, kDownNT, shm, st>>>(d_in, d_out, lm, g, band_only, ws, wsstride,
                                                p->t.D, p->t.T, tab, htab);
   else if (p->precompute)
-    k_down<1, 1, MT><<<gd, kDownNT, shm, st>>>(d_in, d_out, lm, g, band_only, A, ws, wsstride,
+    k_down<1, 1, MT><<<gd, kDownNT, shm, st>>>(d_in, d_out, lm, g, band_only, ws, wsstride,
                                                p->t.D, p->t.T, tab, htab);
   else
-    k_down<1, 0, MT><<<gd, kDownNT, shm, st>>>(d_in, d_out, lm, g, band_only, A, ws, wsstride,
+    k_down<1, 0, MT><<<gd, kDownNT, shm, st>>>(d_in, d_out, lm, g, band_only, ws, wsstride,
                                                p->t.D, p->t.T, tab, htab);
   prof_end(p->prof, st);
 }
@@ -719,7 +902,6 @@ static int launch_fmm(fmm_plan_s *p, const double *d_in, double *d_out, size_t d
   L2C_HIP(hipGetLastError(), -1);
   return 0;
 }
-
 int run_fmm(fmm_plan_s *p, const double *d_in, double *d_out, size_t dir, size_t stride,
             hipStream_t st, int band_only) {
   LineMap lm{1, 0, 0, stride};

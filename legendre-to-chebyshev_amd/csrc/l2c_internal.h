@@ -113,7 +113,7 @@ struct DevTables {
 };
 
 // Per-launch HIP-event timing of the execute kernels (bench support).
-enum KernelId { K_UP_SUBTREE = 0, K_UP_TOP = 1, K_DOWN_BAND = 2, K_DIRECT = 3, K_COUNT = 8 };
+enum KernelId { K_UP_SUBTREE = 0, K_M2L = 1, K_DOWN_BAND = 2, K_DIRECT = 3, K_COUNT = 8 };
 struct Prof {
   int on = 0;
   struct Rec {

@@ -360,15 +360,14 @@ static bool choose_cuts(Geo &g) {
   if (const char *e = getenv("L2C_GU")) gu = atoi(e);
   if (const char *e = getenv("L2C_GD")) gd = atoi(e);
   const size_t budget = 64 * 1024;
-  g.gu = std::max(0, std::min(gu, Lm1));
-  g.gd = std::max(0, std::min(gd, g.gu));  // k_down reads multipoles below lu only
+  g.gu = std::max(0, std::min(std::min(gu, 4), Lm1));  // k_up: at most 16 leaf chunks
+  g.gd = std::max(0, std::min(gd, Lm1));
   for (;;) {
     g.lu = Lm1 - g.gu;
     g.ld = Lm1 - g.gd;
     bool shrink = false;
     if (up_shared_bytes(g) > budget && g.gu > 0) {
       g.gu--;
-      if (g.gd > g.gu) g.gd = g.gu;
       shrink = true;
     } else if (down_shared_bytes(g) > budget && g.gd > 0) {
       g.gd--;
