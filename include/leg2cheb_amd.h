@@ -35,6 +35,8 @@
 #ifdef __cplusplus
 extern "C" {
 #endif
+/* The library is built with hidden visibility; exactly these are exported. */
+#pragma GCC visibility push(default)
 
 /* src/C/leg2cheb.h:27 */
 enum { L2C = 0, C2L = 1, BOTH = 2 };
@@ -166,6 +168,7 @@ int l2c_free_device(void *p);
 int l2c_memcpy_h2d(void *dst, const void *src, size_t bytes);
 int l2c_memcpy_d2h(void *dst, const void *src, size_t bytes);
 
+#pragma GCC visibility pop
 #ifdef __cplusplus
 }
 #endif

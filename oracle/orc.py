@@ -139,12 +139,16 @@ class FMMPlan:
         self.last_flops = f
         return out
 
-    def A(self, direction):
+    def A(self, direction, live=False):
+        """M2L matrices (Nc x M x M).  live=True returns a writable view of the
+        plan's own table (tests plant mistakes through it)."""
         Nc = lib().orc_total_sub(self.L)
-        return np.ctypeslib.as_array(self._p.contents.A[direction], shape=(Nc, self.M, self.M)).copy()
+        v = np.ctypeslib.as_array(self._p.contents.A[direction], shape=(Nc, self.M, self.M))
+        return v if live else v.copy()
 
-    def T(self):
-        return np.ctypeslib.as_array(self._p.contents.T, shape=(2, self.s, self.M)).copy()
+    def T(self, live=False):
+        v = np.ctypeslib.as_array(self._p.contents.T, shape=(2, self.s, self.M))
+        return v if live else v.copy()
 
     def a(self):
         return np.ctypeslib.as_array(self._p.contents.dplan.contents.a, shape=(self.N,)).copy()
