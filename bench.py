@@ -63,14 +63,90 @@ def kernel_bytes(name, info, N):
     return 8 * 2 * N
 
 
-def pmc_traffic(workload):
-    """HBM bytes per k_down_band launch from the committed rocprofv3 --pmc
-    summary of this workload, if present (profiles/<kernel>_<workload>_pmc.json)."""
-    path = os.path.join(ROOT, "profiles", f"k_down_band_{workload}_pmc.json")
+def workload_name(N, maxs, M, d):
+    """Name of the bench workload, with the leaf size s of create_fmm's
+    geometry (src/C/leg2cheb.c:860-871)."""
+    L = math.ceil(math.log2(N / maxs)) - 2
+    s = math.ceil(N / 2 ** (L + 2)) if L >= 1 else N
+    return f"{'l2c' if d == 0 else 'c2l'}_fp64_N{N}_s{s}_M{M}"
+
+
+def pmc_path(workload):
+    return os.path.join(ROOT, "profiles", f"execute_kernels_{workload}_pmc.json")
+
+
+def pmc_traffic(workload, kernel):
+    """HBM bytes per launch of `kernel` from the committed rocprofv3 --pmc
+    summary of this workload (written by --pmc-summary), or None."""
+    path = pmc_path(workload)
     if not os.path.exists(path):
         return None
     with open(path) as f:
-        return json.load(f).get("hbm_bytes_per_launch")
+        k = json.load(f).get("kernels", {}).get(kernel)
+    return None if k is None else k["hbm_bytes_per_launch"]
+
+
+def _counter_means(csv_dir, counter, copy_kib):
+    """Mean per-dispatch value of `counter` for each execute kernel found in
+    the rocprofv3 counter_collection.csv files under csv_dir, and for the
+    calibration copy (copy kernels reading at least half of copy_kib; smaller
+    copies are the input uploads)."""
+    import csv
+    import glob
+    acc = {}
+    for path in glob.glob(os.path.join(csv_dir, "**", "*counter_collection.csv"), recursive=True):
+        with open(path) as f:
+            for r in csv.DictReader(f):
+                if r["Counter_Name"] != counter:
+                    continue
+                name = r["Kernel_Name"]
+                if name.startswith("void "):
+                    name = name[5:]
+                if name.startswith("l2c::"):
+                    name = name[5:]
+                for tag in ("k_up", "k_m2l", "k_down", "k_band", "k_far", "k_direct", "k_lines"):
+                    if name.startswith(tag + "<") or name.startswith(tag + "(") or name == tag:
+                        name = tag
+                        break
+                else:
+                    if "copy" not in name.lower() or float(r["Counter_Value"]) < 0.5 * copy_kib:
+                        continue
+                    name = "copy"
+                s = acc.setdefault(name, [0.0, 0])
+                s[0] += float(r["Counter_Value"])
+                s[1] += 1
+    return {k: v[0] / v[1] for k, v in acc.items()}
+
+
+def pmc_summary(args):
+    """Turn the FETCH_SIZE and WRITE_SIZE passes (KiB per dispatch) into HBM
+    bytes per launch, each scaled by what the same counter reads for the
+    plain copy of --copy-mb MiB run in the same process."""
+    nbytes = args.copy_mb * 1024 * 1024
+    fetch = _counter_means(args.pmc_summary[0], "FETCH_SIZE", nbytes / 1024.0)
+    write = _counter_means(args.pmc_summary[1], "WRITE_SIZE", nbytes / 1024.0)
+    f_scale = nbytes / (fetch["copy"] * 1024.0)
+    w_scale = nbytes / (write["copy"] * 1024.0)
+    out = {"copy_bytes": nbytes, "fetch_scale": f_scale, "write_scale": w_scale,
+           "copy_fetch_kib": fetch["copy"], "copy_write_kib": write["copy"], "kernels": {}}
+    for k in sorted(set(fetch) & set(write)):
+        if k == "copy":
+            continue
+        fb, wb = fetch[k] * 1024.0 * f_scale, write[k] * 1024.0 * w_scale
+        out["kernels"][k] = {"fetch_kib": fetch[k], "write_kib": write[k], "fetch_bytes": round(fb),
+                             "write_bytes": round(wb), "hbm_bytes_per_launch": round(fb + wb)}
+    return out
+
+
+def copy_calibration(amd, mb):
+    """Plain device-to-device copies of `mb` MiB (the runtime's copy kernel),
+    run so the --pmc passes see a kernel of known traffic in the same process."""
+    n = mb * 1024 * 1024 // 8
+    a, b = amd.DeviceBuffer(n), amd.DeviceBuffer(n)  # contents irrelevant
+    for _ in range(3):
+        if amd.lib().l2c_memcpy_d2d(b.ptr, a.ptr, n * 8) != 0:
+            raise RuntimeError(amd.last_error())
+    del a, b
 
 
 def main():
@@ -87,7 +163,25 @@ def main():
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--check-rows", type=int, default=512,
                     help="rows checked against the double-double direct sum")
+    ap.add_argument("--copy-mb", type=int, default=0,
+                    help="run a plain copy of this many MiB first (--pmc calibration)")
+    ap.add_argument("--pmc-summary", nargs=2, metavar=("FETCH_DIR", "WRITE_DIR"),
+                    help="summarise two rocprofv3 --pmc runs (FETCH_SIZE, WRITE_SIZE) of this "
+                         "command into profiles/execute_kernels_<workload>_pmc.json and exit")
     args = ap.parse_args()
+
+    if args.pmc_summary:
+        wl = workload_name(args.N, args.maxs, args.M, args.direction)
+        out = pmc_summary(args)
+        out["workload"] = wl
+        out["command"] = ("rocprofv3 --pmc FETCH_SIZE (then WRITE_SIZE) -- python3 bench.py "
+                          + " ".join(a for a in sys.argv[1:] if a not in args.pmc_summary
+                                     and a != "--pmc-summary"))
+        os.makedirs(os.path.join(ROOT, "profiles"), exist_ok=True)
+        with open(pmc_path(wl), "w") as f:
+            json.dump(out, f, indent=1)
+        print(json.dumps(out, indent=1))
+        return
 
     world = int(os.environ.get("WORLD_SIZE", "1"))
     if args.gpus != 1 or world != 1:
@@ -106,6 +200,10 @@ def main():
     info = plan.info()
     din, dout = amd.DeviceBuffer(N), amd.DeviceBuffer(N)
     din.upload(u)
+
+    if args.copy_mb:
+        log(f"[bench] calibration copy {args.copy_mb} MiB")
+        copy_calibration(amd, args.copy_mb)
 
     log(f"[bench] warmup {args.warmup}")
     for _ in range(args.warmup):
@@ -148,10 +246,11 @@ def main():
     avg_us = kernels[dom]["avg_us"]
     alg = kernel_bytes(dom, info, N)
     achieved = alg / (avg_us * 1e-6) / 1e9
-    workload = f"l2c_fp64_N{N}_s{info['s']}_M{M}" if d == 0 else f"c2l_fp64_N{N}_s{info['s']}_M{M}"
+    workload = workload_name(N, maxs, M, d)
+    assert f"_s{info['s']}_" in workload
     roofline = {"bound": "hbm", "kernel": dom, "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
                 "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
-                "traffic": pmc_traffic(workload), "algorithmic_bytes": alg,
+                "traffic": pmc_traffic(workload, dom), "algorithmic_bytes": alg,
                 "avg_launch_us": round(avg_us, 3)}
 
     cpu = None

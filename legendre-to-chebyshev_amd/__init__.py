@@ -105,6 +105,8 @@ def lib():
         L.l2c_memcpy_h2d.argtypes = [vp, vp, _sz]
         L.l2c_memcpy_d2h.restype = C.c_int
         L.l2c_memcpy_d2h.argtypes = [vp, vp, _sz]
+        L.l2c_memcpy_d2d.restype = C.c_int
+        L.l2c_memcpy_d2d.argtypes = [vp, vp, _sz]
         _lib = L
     return _lib
 

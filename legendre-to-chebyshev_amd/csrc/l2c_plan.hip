@@ -752,6 +752,10 @@ int l2c_memcpy_d2h(void *dst, const void *src, size_t bytes) {
   L2C_HIP(hipMemcpy(dst, src, bytes, hipMemcpyDeviceToHost), -1);
   return 0;
 }
+int l2c_memcpy_d2d(void *dst, const void *src, size_t bytes) {
+  L2C_HIP(hipMemcpy(dst, src, bytes, hipMemcpyDeviceToDevice), -1);
+  return 0;
+}
 int l2c_synchronize(fmm_plan *p) {
   if (!p) return -1;
   L2C_HIP(hipStreamSynchronize(p->stream), -1);
