@@ -104,9 +104,11 @@ def _counter_means(csv_dir, counter, copy_kib):
                     name = name[5:]
                 if name.startswith("l2c::"):
                     name = name[5:]
-                for tag in ("k_up", "k_m2l", "k_down", "k_band", "k_far", "k_direct", "k_lines"):
+                # profile slots: k_down covers both downward kernels (k_down8, k_down)
+                for tag, slot in (("k_up", "k_up"), ("k_m2l", "k_m2l"), ("k_down8", "k_down"),
+                                  ("k_down", "k_down"), ("k_direct", "k_direct")):
                     if name.startswith(tag + "<") or name.startswith(tag + "(") or name == tag:
-                        name = tag
+                        name = slot
                         break
                 else:
                     if "copy" not in name.lower() or float(r["Counter_Value"]) < 0.5 * copy_kib:
@@ -163,6 +165,8 @@ def main():
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--check-rows", type=int, default=512,
                     help="rows checked against the double-double direct sum")
+    ap.add_argument("--prof-every", type=int, default=16,
+                    help="time the kernels of every n-th step with HIP events")
     ap.add_argument("--copy-mb", type=int, default=0,
                     help="run a plain copy of this many MiB first (--pmc calibration)")
     ap.add_argument("--pmc-summary", nargs=2, metavar=("FETCH_DIR", "WRITE_DIR"),
@@ -211,7 +215,9 @@ def main():
     plan.synchronize()
 
     log(f"[bench] timed {args.steps} steps")
-    plan.profile(True)
+    # kernel durations from HIP events around the kernels of every
+    # --prof-every-th step (events between kernels cost GPU time themselves)
+    plan.profile(args.prof_every)
     plan.synchronize()
     t0 = time.perf_counter()
     for _ in range(args.steps):

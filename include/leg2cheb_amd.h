@@ -23,7 +23,7 @@
  *     plan must not be executed from two threads at once.
  *
  * Extensions (l2c_*) expose device-resident execution, batched and strided
- * line execution, the split-precision form and plan introspection.
+ * line execution, per-kernel timing and plan introspection.
  */
 #ifndef LEG2CHEB_AMD_H
 #define LEG2CHEB_AMD_H
@@ -146,16 +146,17 @@ int l2c_execute_lines_device(fmm_plan *plan, const double *d_in, double *d_out,
                              size_t line_stride0, size_t line_stride1,
                              size_t elem_stride, void *stream);
 
-/* Precision form of a plan: 0 = plain FP64 (default); 1 = split precision
- * (M2L and leaf products as BF16-sliced MFMA products, FP32 accumulation,
- * FP64 recombination; Lambda tables, near-field band and DCT stay FP64). */
+/* Precision form of a plan.  0 = plain FP64 (default and, in this build,
+ * the only form: any other value returns -1 with an error text). */
 int l2c_set_precision(fmm_plan *plan, int form);
 
 /* Per-kernel timing with HIP events recorded on the launch stream around
- * every execute kernel.  l2c_profile(plan, 1) resets and enables it;
- * l2c_profile_read fills ms[8] (summed milliseconds) and counts[8] (launches)
- * indexed 0 k_up, 1 k_m2l, 2 k_down, 3 k_direct,
- * 4 k_batch, 5 k_split_m2l, 6 k_lines_direct, 7 reserved. */
+ * the execute kernels.  l2c_profile(plan, n) resets the sums and, for n >= 1,
+ * times the kernels of every n-th call (execute, directM, lines); n = 0 turns
+ * it off.  Events between kernels cost several microseconds of GPU time each,
+ * so a period n > 1 keeps the measured throughput close to the unprofiled
+ * one.  l2c_profile_read fills ms[8] (summed milliseconds) and counts[8]
+ * (timed launches) indexed 0 k_up, 1 k_m2l, 2 k_down, 3 k_direct, 4-7 unused. */
 int l2c_profile(fmm_plan *plan, int enable);
 int l2c_profile_read(fmm_plan *plan, double *ms, size_t *counts);
 

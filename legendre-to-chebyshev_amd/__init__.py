@@ -224,11 +224,15 @@ class FMMPlan:
         if lib().l2c_synchronize(self._plan) != 0:
             raise L2CError(last_error())
 
-    KERNELS = ("k_up", "k_m2l", "k_down", "k_direct", "k_batch",
-               "k_split_m2l", "k_lines_direct", "reserved")
+    # execute kernels by profile slot
+    KERNELS = ("k_up", "k_m2l", "k_down", "k_direct", "reserved4", "reserved5", "reserved6",
+               "reserved7")
 
     def profile(self, enable=True):
-        if lib().l2c_profile(self._plan, 1 if enable else 0) != 0:
+        """Time the execute kernels with HIP events: True (every call), False
+        (off) or an int n (every n-th call)."""
+        n = int(enable) if not isinstance(enable, bool) else (1 if enable else 0)
+        if lib().l2c_profile(self._plan, n) != 0:
             raise L2CError(last_error())
 
     def profile_read(self):

@@ -17,8 +17,13 @@ namespace l2c {
 
 // ---------------------------------------------------------------------------
 // per-launch event timing
-void prof_begin(Prof *pf, int id, hipStream_t st) {
+void prof_step(Prof *pf) {
   if (!pf || !pf->on) return;
+  pf->cur = pf->calls++ % (size_t)pf->on == 0;
+}
+
+void prof_begin(Prof *pf, int id, hipStream_t st) {
+  if (!pf || !pf->on || !pf->cur) return;
   if (pf->used + 2 > pf->pool.size()) {
     for (int i = 0; i < 256; i++) {
       hipEvent_t e;
@@ -33,7 +38,7 @@ void prof_begin(Prof *pf, int id, hipStream_t st) {
 }
 
 void prof_end(Prof *pf, hipStream_t st) {
-  if (!pf || !pf->on || pf->recs.empty()) return;
+  if (!pf || !pf->on || !pf->cur || pf->recs.empty()) return;
   (void)hipEventRecord(pf->recs.back().b, st);
 }
 
@@ -260,7 +265,8 @@ int l2c_profile(fmm_plan *p, int enable) {
   }
   if (!p->prof) p->prof = new Prof();
   if (p->prof->on) prof_harvest(p->prof);
-  p->prof->on = enable;
+  p->prof->on = enable > 0 ? enable : 0;
+  p->prof->calls = 0;
   for (int i = 0; i < K_COUNT; i++) {
     p->prof->ms[i] = 0;
     p->prof->cnt[i] = 0;

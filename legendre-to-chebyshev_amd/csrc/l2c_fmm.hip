@@ -56,6 +56,16 @@ size_t ws_doubles(const Geo &g) { return ws_layout(g).total; }
 __host__ __device__ __forceinline__ int sk4(int c) { return c + (c >> 2); }
 __host__ __device__ __forceinline__ int sk8(int k) { return k + (k >> 3); }
 
+// Work-group barrier for LDS traffic only: it orders LDS reads and writes
+// like __syncthreads() but does not wait for global loads and stores still
+// in flight, so prefetches and result stores overlap the next phase.  Use it
+// only where the threads exchange data through LDS alone.
+__device__ __forceinline__ void lds_sync() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup", "local");
+  __builtin_amdgcn_s_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup", "local");
+}
+
 // ---------------------------------------------------------------------------
 // k_up: grid (2^(lu+2), lines), 256 threads.  G = 2^gu <= 16 leaf chunks.
 //
@@ -104,6 +114,20 @@ __global__ void __launch_bounds__(kUpNT) k_up(const double *__restrict__ in, Lin
     const size_t i = base + e;
     vin[q] = (e < nin && i < (size_t)g.N) ? src[i * lm.es] : 0.0;
   }
+  // B operand of this wave's first MFMA task (16 k-steps = the first 64 rows of T)
+  const int ntile = (M + 15) / 16;
+  constexpr int QT = 16;
+  double tb[QT];
+  {
+    const int sg = wave / ntile, nt = wave % ntile;
+    const int kb = nt * 16 + (lane & 15), ka = lane >> 4;
+    const double *tcol = T + (size_t)sg * s * M + kb;
+#pragma unroll
+    for (int q = 0; q < QT; q++) {
+      const int m = 4 * q + ka;
+      tb[q] = (wave < 2 * ntile && m < s && kb < M) ? tcol[(size_t)m * M] : 0.0;
+    }
+  }
   const int ku = tid % M;
   double u0r[MT > 0 ? MT : 1], u1r[MT > 0 ? MT : 1];
   const int np = g.lu * M2;
@@ -151,7 +175,6 @@ __global__ void __launch_bounds__(kUpNT) k_up(const double *__restrict__ in, Lin
   __syncthreads();
 
   // ---- step 1 on MFMA: W_par[i][k] = sum_m X_par[i][m] T_par[m][k]
-  const int ntile = (M + 15) / 16;
   for (int task = wave; task < 2 * ntile; task += kUpNT / 64) {
     const int sg = task / ntile, nt = task % ntile;
     const int ia = lane & 15, ka = lane >> 4;  // A: row i, k-offset
@@ -159,7 +182,17 @@ __global__ void __launch_bounds__(kUpNT) k_up(const double *__restrict__ in, Lin
     const double *xrow = xs + sg * 16 * xst + ia * xst;
     const double *tcol = T + (size_t)sg * s * M + kb;
     d4 acc = {0, 0, 0, 0};
-    for (int m0 = 0; m0 < s; m0 += 4) {
+    int m0 = 0;
+    if (task == wave) {  // B prefetched for rows < 64
+#pragma unroll
+      for (int q = 0; q < QT; q++) {
+        const int m = 4 * q + ka;
+        const double a = m < s ? xrow[m] : 0.0;
+        acc = __builtin_amdgcn_mfma_f64_16x16x4f64(a, tb[q], acc, 0, 0, 0);
+      }
+      m0 = 4 * QT;
+    }
+    for (; m0 < s; m0 += 4) {
       const int m = m0 + ka;
       const double a = m < s ? xrow[m] : 0.0;
       const double b = (m < s && kb < M) ? tcol[(size_t)m * M] : 0.0;
@@ -421,6 +454,10 @@ __global__ void __launch_bounds__(kM2lNT) k_m2l(Geo g, const double *__restrict_
 // xsg: parity's input window (skew sk4, origin at the work-group's first
 // half-row), aw: Lambda window (skew sk8, origin at row0), hd: band head.
 // ALIGNED: rl0 % 4 == 0, so every LDS offset inside a trip is a constant.
+// Software-pipelined: the LDS reads of trip b+1 (eight x values, eight head
+// values, eight ring refills) are issued before the arithmetic of trip b, so
+// LDS latency hides behind 64 FP64 operations.  Reads run at most one trip
+// (eight columns) past len, which stays inside the zero-filled windows.
 template <int DIR, int PRE, int SG, int ALIGNED>
 __device__ __forceinline__ void band_rows(const double *__restrict__ xsg,
                                           const double *__restrict__ aw,
@@ -445,6 +482,7 @@ __device__ __forceinline__ void band_rows(const double *__restrict__ xsg,
   double m[4] = {0, 0, 0, 0};
   double p[4] = {pbp[OP[0]], pbp[OP[1]], pbp[OP[2]], pbp[OP[3]]};
   double a[4] = {acc[0], acc[1], acc[2], acc[3]};
+  double XA[8], HA[8], PA[8], XB[8], HB[8], PB[8];
 #define L2C_TERM(am, ap, x, acc_)   \
   if (DIR == 0)                     \
     acc_ += ((am) * (ap)) * (x);    \
@@ -452,31 +490,55 @@ __device__ __forceinline__ void band_rows(const double *__restrict__ xsg,
     acc_ -= ((am) * (ap)) * (x);    \
   else                              \
     acc_ -= ((am) * (x)) * (ap);
-#define BAND_STEP(K, XV)                                  \
-  {                                                       \
-    m[(K) & 3] = hb[K];                                   \
-    const double x_ = (XV);                               \
-    L2C_TERM(m[(K) & 3], p[(K) & 3], x_, a[0])            \
+#define BAND_STEP(K, XV, HV, PV)                           \
+  {                                                        \
+    m[(K) & 3] = (HV);                                     \
+    const double x_ = (XV);                                \
+    L2C_TERM(m[(K) & 3], p[(K) & 3], x_, a[0])             \
     L2C_TERM(m[((K) + 3) & 3], p[((K) + 1) & 3], x_, a[1]) \
     L2C_TERM(m[((K) + 2) & 3], p[((K) + 2) & 3], x_, a[2]) \
     L2C_TERM(m[((K) + 1) & 3], p[((K) + 3) & 3], x_, a[3]) \
-    p[(K) & 3] = pbp[OP[(K) + 4]];                        \
+    p[(K) & 3] = (PV);                                     \
   }
+  // trip t's data: x at xb + 10t, heads at hb + 8t, refills at pbp + 9t
+#define LOAD_TRIP(X, H, P, T)                                      \
+  _Pragma("unroll") for (int k_ = 0; k_ < 8; k_++) {               \
+    X[k_] = xb[10 * (T) + OX[k_]];                                 \
+    H[k_] = hb[8 * (T) + k_];                                      \
+    P[k_] = pbp[9 * (T) + OP[k_ + 4]];                             \
+  }
+#define RUN_TRIP(X, H, P)                                                  \
+  BAND_STEP(0, X[0], H[0], P[0]) BAND_STEP(1, X[1], H[1], P[1])            \
+  BAND_STEP(2, X[2], H[2], P[2]) BAND_STEP(3, X[3], H[3], P[3])            \
+  BAND_STEP(4, X[4], H[4], P[4]) BAND_STEP(5, X[5], H[5], P[5])            \
+  BAND_STEP(6, X[6], H[6], P[6]) BAND_STEP(7, X[7], H[7], P[7])
   const int nb = len >> 3, rem = len & 7;
-  for (int b = 0; b < nb; b++) {
-    BAND_STEP(0, xb[OX[0]]) BAND_STEP(1, xb[OX[1]]) BAND_STEP(2, xb[OX[2]])
-    BAND_STEP(3, xb[OX[3]]) BAND_STEP(4, xb[OX[4]]) BAND_STEP(5, xb[OX[5]])
-    BAND_STEP(6, xb[OX[6]]) BAND_STEP(7, xb[OX[7]])
-    xb += 10;
-    pbp += 9;
-    hb += 8;
+  LOAD_TRIP(XA, HA, PA, 0)
+  int b = 0;
+  for (; b + 2 <= nb; b += 2) {
+    LOAD_TRIP(XB, HB, PB, 1)
+    RUN_TRIP(XA, HA, PA)
+    LOAD_TRIP(XA, HA, PA, 2)
+    RUN_TRIP(XB, HB, PB)
+    xb += 20;
+    pbp += 18;
+    hb += 16;
+  }
+  if (b < nb) {
+    RUN_TRIP(XA, HA, PA)
+    LOAD_TRIP(XA, HA, PA, 1)
   }
   if (rem) {
-    BAND_STEP(0, xb[OX[0]]) BAND_STEP(1, rem > 1 ? xb[OX[1]] : 0.0)
-    BAND_STEP(2, rem > 2 ? xb[OX[2]] : 0.0) BAND_STEP(3, rem > 3 ? xb[OX[3]] : 0.0)
-    BAND_STEP(4, rem > 4 ? xb[OX[4]] : 0.0) BAND_STEP(5, rem > 5 ? xb[OX[5]] : 0.0)
-    BAND_STEP(6, rem > 6 ? xb[OX[6]] : 0.0)
+    BAND_STEP(0, XA[0], HA[0], PA[0])
+    BAND_STEP(1, rem > 1 ? XA[1] : 0.0, HA[1], PA[1])
+    BAND_STEP(2, rem > 2 ? XA[2] : 0.0, HA[2], PA[2])
+    BAND_STEP(3, rem > 3 ? XA[3] : 0.0, HA[3], PA[3])
+    BAND_STEP(4, rem > 4 ? XA[4] : 0.0, HA[4], PA[4])
+    BAND_STEP(5, rem > 5 ? XA[5] : 0.0, HA[5], PA[5])
+    BAND_STEP(6, rem > 6 ? XA[6] : 0.0, HA[6], PA[6])
   }
+#undef RUN_TRIP
+#undef LOAD_TRIP
 #undef BAND_STEP
 #undef L2C_TERM
   acc[0] = a[0];
@@ -509,9 +571,10 @@ constexpr int kR = 4;  // band rows per register block
 constexpr int kDownNT = 128;
 
 // LDS plan of k_down (offsets in doubles).  The M2L vectors and the chain
-// buffer are dead once the band phase starts; the output staging reuses them.
+// partial sums are dead once the band phase starts; the output staging
+// reuses them.
 struct DownLds {
-  int xs, aw, hd, cres, mres, cb, out, total;
+  int xs, aw, cres, mres, qp, out, total;
 };
 __host__ __device__ inline DownLds down_lds(const Geo &g) {
   const int G = 1 << g.gd, s = g.s, M2 = 2 * g.M;
@@ -522,28 +585,218 @@ __host__ __device__ inline DownLds down_lds(const Geo &g) {
   const int nas = na + (na >> 3) + 2;
   d.xs = 0;
   d.aw = d.xs + 2 * nxs;
-  d.hd = d.aw + nas;
-  d.cres = d.hd + 2 * s + 16;
+  d.cres = d.aw + nas;
   d.mres = d.cres + (2 * G - 1) * M2;
-  d.cb = d.mres + (g.ld + 2 * G - 1) * M2;
+  d.qp = d.mres + (g.ld + 2 * G - 1) * M2;
   d.out = d.mres;
-  d.total = std::max(d.cb + 2 * M2, d.out + 2 * G * s);
+  d.total = std::max(d.qp + std::max(g.ld, 1) * M2, d.out + 2 * G * s);
   return d;
 }
-size_t down_shared_bytes(const Geo &g) { return 8 * (size_t)down_lds(g).total; }
+size_t down8_shared_bytes(const Geo &g);
+size_t down_shared_bytes(const Geo &g) {
+  return g.fast ? down8_shared_bytes(g) : 8 * (size_t)down_lds(g).total;
+}
 
-// Subtree chunk slot cl (level ld + lam, position ci) of work-group Rr.
+// Subtree chunk slot cl (level ld + lam, position ci) of work-group Rr:
+// slots are level-major, level ld + lam starting at 2^lam - 1.
 __device__ __forceinline__ void sub_slot(int cl, int &lam, int &ci) {
-  lam = 0;
-  while ((2 << lam) - 1 <= cl) lam++;
-  ci = cl - ((1 << lam) - 1);
+  lam = 31 - __builtin_clz(cl + 1);
+  ci = cl + 1 - (1 << lam);
+}
+
+// Address of M2L vector element e of the work-group's list [chain levels
+// 0..ld-1 (ancestors of chunk Rr at level ld)][subtree slots], and whether the
+// chunk has an M2L block (the last two chunks of every level have none).
+__device__ __forceinline__ size_t m2l_elem(const Geo &g, int Rr, int e, int M2, bool &valid) {
+  const int slot = e / M2, o = e - slot * M2;
+  int lev;
+  size_t r;
+  if (slot < g.ld) {
+    lev = slot;
+    r = (size_t)Rr >> (g.ld - lev);
+  } else {
+    int lam, ci;
+    sub_slot(slot - g.ld, lam, ci);
+    lev = g.ld + lam;
+    r = ((size_t)Rr << lam) + ci;
+  }
+  valid = r < ((size_t)4 << lev) - 2;
+  return (slot_base(lev) + r) * M2 + o;
+}
+
+// Row k of the composite operator Q(ld - lev, Rr mod 2^(ld - lev)) for chain
+// task (lev, k) = (task / M, task % M).
+__device__ __forceinline__ const double *qrow_ptr(const double *Qg, const Geo &g, int Rr, int task,
+                                                  int M) {
+  const int lev = task / M, k = task - lev * M, kk = g.ld - lev;
+  const size_t d = (size_t)Rr & (((size_t)1 << kk) - 1);
+  return Qg + ((((size_t)1 << kk) - 2) + d) * (size_t)(M * M) + (size_t)k * M;
+}
+
+// Local expansions (L2L, src/C/leg2cheb.c:1437-1449) of the work-group's
+// subtree into cres [slot][parity][M]: the root's c_ld = sum_lev Q(ld - lev,
+// .) m2l_lev + m2l_ld replaces the chain c_l = D(r_l & 1) c_(l-1) + m2l_l
+// (chunks without an M2L block, the last two of a level and all their
+// ancestors, contribute zero exactly as in the chain); then levels
+// ld+1..ld+gd: c = D(r & 1) c_parent + m2l.  mres: the M2L vectors in
+// m2l_elem order; qp: ld x 2M scratch.  qr: row (tid / M, tid % M) of the
+// chain operator when MT > 0 (else read from Qg).  DLDS: the subtree levels
+// run on MFMA with D(p) from the LDS copy Dl ([p][k][M + 1], padded rows);
+// else on VALU with D(p) row tid % M in d0r/d1r (MT > 0) or from Dg.  Every
+// thread of the work-group must call this.
+template <int MT, bool DLDS>
+__device__ __forceinline__ void subtree_locals(const Geo &g, int Rr, int tid, int NT,
+                                               const double *mres, double *qp, double *cres,
+                                               const double *qr, const double *d0r,
+                                               const double *d1r, const double *Qg,
+                                               const double *Dg, const double *Dl) {
+  const int M = MT > 0 ? MT : g.M, M2 = 2 * M, MM = M * M;
+  const int nq = g.ld * M;
+  int task0 = tid;
+  if (MT > 0 && tid < nq) {  // first task: the operator row is in registers
+    const int lev = tid / M, k = tid - lev * M;
+    const double *mv = mres + lev * M2;
+    double o0 = 0, o1 = 0;
+#pragma unroll
+    for (int l = 0; l < (MT > 0 ? MT : 1); l++) {
+      o0 += qr[l] * mv[l];
+      o1 += qr[l] * mv[M + l];
+    }
+    qp[lev * M2 + k] = o0;
+    qp[lev * M2 + M + k] = o1;
+    task0 += NT;
+  }
+  for (int task = task0; task < nq; task += NT) {
+    const int lev = task / M, k = task - lev * M;
+    const double *mv = mres + lev * M2;
+    const double *qrow = qrow_ptr(Qg, g, Rr, task, M);
+    double o0 = 0, o1 = 0;
+    for (int l = 0; l < M; l++) {
+      o0 += qrow[l] * mv[l];
+      o1 += qrow[l] * mv[M + l];
+    }
+    qp[lev * M2 + k] = o0;
+    qp[lev * M2 + M + k] = o1;
+  }
+  lds_sync();
+  for (int o = tid; o < M2; o += NT) {
+    // fixed trip count so the partial-sum reads issue together (ld < 28)
+    double v = mres[g.ld * M2 + o];
+    const int lmax = g.ld > 0 ? g.ld - 1 : 0;
+#pragma unroll
+    for (int lev = 0; lev < 28; lev++) {
+      const double q = qp[std::min(lev, lmax) * M2 + o];
+      v += lev < g.ld ? q : 0.0;
+    }
+    cres[o] = v;
+  }
+  lds_sync();
+  if (DLDS) {
+    // levels ld+1..ld+gd on v_mfma_f64_16x16x4f64: children of parity p are
+    // C (M x n) = D(p) (M x M) C_parents (M x n), n = 2^lam children x 2
+    // parities <= 16 columns: column j = child 2(j >> 1) + p, parity j & 1.
+    // Work item (p, 16-row tile rt) per wave.
+    const int lane = tid & 63, wave = __builtin_amdgcn_readfirstlane(tid >> 6), nw = NT >> 6;
+    const int nrt = (M + 15) >> 4, ksteps = (M + 3) >> 2;
+    const int j = lane & 15;
+    for (int lam = 1; lam <= g.gd; lam++) {
+      const int n = 1 << lam, lev = g.ld + lam;
+      const int b0 = n - 1, bp = (n >> 1) - 1;
+      for (int item = wave; item < 2 * nrt; item += nw) {
+        const int p = item / nrt, rt = item - p * nrt;
+        const int ra = rt * 16 + j;  // A row
+        d4 D = {0, 0, 0, 0};
+        if (MT > 0) {  // all operand reads first, then the MFMA chain
+          constexpr int QK = MT > 0 ? (MT + 3) / 4 : 1;
+          double av[QK], bv[QK];
+#pragma unroll
+          for (int kk = 0; kk < QK; kk++) {
+            const int k = kk * 4 + (lane >> 4);
+            av[kk] = (ra < M && k < M) ? Dl[(p * M + ra) * (M + 1) + k] : 0.0;
+            bv[kk] = (j < n && k < M) ? cres[(bp + (j >> 1)) * M2 + (j & 1) * M + k] : 0.0;
+          }
+#pragma unroll
+          for (int kk = 0; kk < QK; kk++) D = __builtin_amdgcn_mfma_f64_16x16x4f64(av[kk], bv[kk], D, 0, 0, 0);
+        } else {
+          for (int kk = 0; kk < ksteps; kk++) {
+            const int k = kk * 4 + (lane >> 4);
+            const double a = (ra < M && k < M) ? Dl[(p * M + ra) * (M + 1) + k] : 0.0;
+            const double b = (j < n && k < M) ? cres[(bp + (j >> 1)) * M2 + (j & 1) * M + k] : 0.0;
+            D = __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, D, 0, 0, 0);
+          }
+        }
+        const int c = 2 * (j >> 1) + p, sgc = j & 1;
+        const bool valid = j < n && ((size_t)Rr << lam) + c < ((size_t)4 << lev) - 2;
+#pragma unroll
+        for (int v = 0; v < 4; v++) {
+          const int row = rt * 16 + (lane >> 4) + 4 * v;
+          if (row < M && j < n) {
+            const int o = (b0 + c) * M2 + sgc * M + row;
+            cres[o] = valid ? D[v] + mres[g.ld * M2 + o] : 0.0;
+          }
+        }
+      }
+      lds_sync();
+    }
+    return;
+  }
+  const int kd = tid % M, ngrp = NT / M2;
+  if (ngrp == 0) {  // fewer threads than 2M: one output per thread and pass, D from memory
+    for (int lam = 1; lam <= g.gd; lam++) {
+      const int n = 1 << lam, lev = g.ld + lam;
+      const int b0 = n - 1, bp = (n >> 1) - 1;
+      for (int idx = tid; idx < n * M2; idx += NT) {
+        const int ci = idx / M2, oo = idx - ci * M2, sg = oo / M, k = oo - sg * M;
+        const size_t r = ((size_t)Rr << lam) + ci;
+        double v = 0;
+        if (r < ((size_t)4 << lev) - 2) {
+          const double *drow = Dg + (r & 1) * MM + (size_t)k * M;
+          const double *par = cres + (bp + (ci >> 1)) * M2 + sg * M;
+          for (int l = 0; l < M; l++) v += drow[l] * par[l];
+          v += mres[(g.ld + b0 + ci) * M2 + oo];
+        }
+        cres[(b0 + ci) * M2 + oo] = v;
+      }
+      lds_sync();
+    }
+    return;
+  }
+  auto l2l = [&](int p, const double *par) -> double {
+    double a0 = 0, a1 = 0;
+    if (MT > 0) {
+#pragma unroll
+      for (int l = 0; l < (MT > 0 ? MT : 1); l += 2) {
+        a0 += (p ? d1r[l] : d0r[l]) * par[l];
+        if (l + 1 < M) a1 += (p ? d1r[l + 1] : d0r[l + 1]) * par[l + 1];
+      }
+    } else {
+      for (int l = 0; l < M; l++) a0 += Dg[p * MM + kd * M + l] * par[l];
+    }
+    return a0 + a1;
+  };
+  for (int lam = 1; lam <= g.gd; lam++) {
+    const int n = 1 << lam, lev = g.ld + lam;
+    const int b0 = n - 1, bp = (n >> 1) - 1;
+    if (tid < ngrp * M2) {
+      const int oo = tid % M2, sg = oo / M;
+      for (int ci = tid / M2; ci < n; ci += ngrp) {
+        const size_t r = ((size_t)Rr << lam) + ci;
+        double v = 0;
+        if (r < ((size_t)4 << lev) - 2)
+          v = l2l((int)(r & 1), cres + (bp + (ci >> 1)) * M2 + sg * M) +
+              mres[(g.ld + b0 + ci) * M2 + oo];
+        cres[(b0 + ci) * M2 + oo] = v;
+      }
+    }
+    lds_sync();
+  }
 }
 
 template <int DIR, int PRE, int MT>
-__global__ void __launch_bounds__(kDownNT) k_down(
+__global__ void __launch_bounds__(kDownNT) __attribute__((amdgpu_waves_per_eu(2))) k_down(
     const double *__restrict__ in, double *__restrict__ out, LineMap lm, Geo g, int band_only,
     const double *__restrict__ wsall, size_t wsstride, const double *__restrict__ Dg,
-    const double *__restrict__ Tg, const double *__restrict__ tab,
+    const double *__restrict__ Qg, const double *__restrict__ Tg, const double *__restrict__ tab,
     const double *__restrict__ htab) {
   extern __shared__ double sm[];
   const int M = MT > 0 ? MT : g.M;
@@ -564,74 +817,52 @@ __global__ void __launch_bounds__(kDownNT) k_down(
   double *xs = sm + ld.xs;
   const int nxs = (ld.aw - ld.xs) / 2;
   double *aw = sm + ld.aw;
-  double *hd = sm + ld.hd;
   double *cres = sm + ld.cres;
   double *mres = sm + ld.mres;
-  double *cb = sm + ld.cb;
+  double *qp = sm + ld.qp;
   double *ob = sm + ld.out;
 
-  // ---- phase 0: every load issued before any store
+  // ---- phase 0: every load issued before any store; addresses are clamped
+  // into range and out-of-range values replaced afterwards, so the loads
+  // issue as one straight-line batch
   const int nwin = G * 2 * s + 2 * s + 32;
   constexpr int QW = 10;
   double vx[QW], va[QW];
 #pragma unroll
   for (int q = 0; q < QW; q++) {
-    const int e = q * NT + tid;
-    const size_t i = row0 + e;
-    const bool ok = e < nwin && i < (size_t)N;
-    vx[q] = ok ? src[i * lm.es] : 0.0;
-    va[q] = ok ? tab[i] : 0.0;
-  }
-  const int nh = std::min(2 * s, DIR == 0 ? N : (N + 1) / 2);
-  const int nhd = 2 * s + 16;
-  double vh[3];
-#pragma unroll
-  for (int q = 0; q < 3; q++) {
-    const int e = q * NT + tid;
-    vh[q] = e < nh ? htab[e] : 0.0;
+    const size_t i = std::min(row0 + q * NT + tid, (size_t)N - 1);
+    vx[q] = src[i * lm.es];
+    va[q] = tab[i];
   }
   // M2L vectors: the ancestor chain (levels 0..ld-1) then the subtree slots
   const int nm = band_only ? 0 : (g.ld + 2 * G - 1) * M2;
   constexpr int QM = 8;
   double vm[QM];
+  bool okm[QM];
 #pragma unroll
   for (int q = 0; q < QM; q++) {
     const int e = q * NT + tid;
-    double v = 0.0;
-    if (e < nm) {
-      const int slot = e / M2, o = e % M2;
-      int lev;
-      size_t r;
-      if (slot < g.ld) {
-        lev = slot;
-        r = (size_t)Rr >> (g.ld - lev);
-      } else {
-        int lam, ci;
-        sub_slot(slot - g.ld, lam, ci);
-        lev = g.ld + lam;
-        r = ((size_t)Rr << lam) + ci;
-      }
-      if (r < ((size_t)4 << lev) - 2) v = m2l[(slot_base(lev) + r) * M2 + o];
-    }
-    vm[q] = v;
+    const size_t a = m2l_elem(g, Rr, e < nm ? e : 0, M2, okm[q]);
+    okm[q] = okm[q] && e < nm;
+    vm[q] = m2l[okm[q] ? a : 0];
   }
-  // this thread's rows of the two L2L operators (k = tid % M)
-  const int kd = tid % M;
-  double d0r[MT > 0 ? MT : 1], d1r[MT > 0 ? MT : 1];
-  if (MT > 0 && !band_only) {
+  // ancestor chain in one step: task (lev, k) < ld*M takes row k of the
+  // composite operator Q(ld - lev, Rr mod 2^(ld - lev)) from level lev to ld
+  const int nq = band_only ? 0 : g.ld * M;
+  double qr[MT > 0 ? MT : 1];
+  if (MT > 0 && nq > 0) {
+    const double *qrow = qrow_ptr(Qg, g, Rr, tid < nq ? tid : 0, M);
 #pragma unroll
-    for (int l = 0; l < (MT > 0 ? MT : 1); l++) {
-      d0r[l] = Dg[kd * M + l];
-      d1r[l] = Dg[MM + kd * M + l];
-    }
+    for (int l = 0; l < (MT > 0 ? MT : 1); l++) qr[l] = qrow[l];
   }
 #pragma unroll
   for (int q = 0; q < QW; q++) {
     const int e = q * NT + tid;
     if (e < nwin) {
       const size_t i = row0 + e;
-      xs[(e & 1) * nxs + sk4(e >> 1)] = (DIR == 1 && i > 0) ? vx[q] * (double)i : vx[q];  // v_i = i u_i
-      aw[sk8(e)] = va[q];
+      const double u = i < (size_t)N ? vx[q] : 0.0;
+      xs[(e & 1) * nxs + sk4(e >> 1)] = (DIR == 1 && i > 0) ? u * (double)i : u;  // v_i = i u_i
+      aw[sk8(e)] = i < (size_t)N ? va[q] : 0.0;
     }
   }
   for (int e = QW * NT + tid; e < nwin; e += NT) {
@@ -642,88 +873,31 @@ __global__ void __launch_bounds__(kDownNT) k_down(
     aw[sk8(e)] = i < (size_t)N ? tab[i] : 0.0;
   }
 #pragma unroll
-  for (int q = 0; q < 3; q++) {
-    const int e = q * NT + tid;
-    if (e < nhd) hd[e] = vh[q];
-  }
-  for (int e = 3 * NT + tid; e < nhd; e += NT) hd[e] = e < nh ? htab[e] : 0.0;
-#pragma unroll
   for (int q = 0; q < QM; q++) {
     const int e = q * NT + tid;
-    if (e < nm) mres[e] = vm[q];
+    if (e < nm) mres[e] = okm[q] ? vm[q] : 0.0;
   }
   for (int e = QM * NT + tid; e < nm; e += NT) {
-    const int slot = e / M2, o = e % M2;
-    int lev;
-    size_t r;
-    if (slot < g.ld) {
-      lev = slot;
-      r = (size_t)Rr >> (g.ld - lev);
-    } else {
-      int lam, ci;
-      sub_slot(slot - g.ld, lam, ci);
-      lev = g.ld + lam;
-      r = ((size_t)Rr << lam) + ci;
-    }
-    mres[e] = r < ((size_t)4 << lev) - 2 ? m2l[(slot_base(lev) + r) * M2 + o] : 0.0;
+    bool ok;
+    const size_t a = m2l_elem(g, Rr, e, M2, ok);
+    mres[e] = ok ? m2l[a] : 0.0;
   }
-  if (!band_only && tid < M2) cb[tid] = 0.0;
+  // this thread's rows of the two L2L operators (k = tid % M), needed from
+  // the subtree pass on; loaded after the window registers are released
+  const int kd = tid % M;
+  double d0r[MT > 0 ? MT : 1], d1r[MT > 0 ? MT : 1];
+  if (MT > 0 && !band_only) {
+#pragma unroll
+    for (int l = 0; l < (MT > 0 ? MT : 1); l++) {
+      d0r[l] = Dg[kd * M + l];
+      d1r[l] = Dg[MM + kd * M + l];
+    }
+  }
   __syncthreads();
 
-  // ---- phase 2: L2L (src/C/leg2cheb.c:1437-1449) down the ancestor chain,
-  // the root and the subtree: c = D(p) c_parent + m2l
-  auto l2l = [&](int p, const double *par) -> double {
-    double a0 = 0, a1 = 0;
-    if (MT > 0) {
-#pragma unroll
-      for (int l = 0; l < (MT > 0 ? MT : 1); l += 2) {
-        a0 += (p ? d1r[l] : d0r[l]) * par[l];
-        if (l + 1 < M) a1 += (p ? d1r[l + 1] : d0r[l + 1]) * par[l + 1];
-      }
-    } else {
-      for (int l = 0; l < M; l++) a0 += Dg[p * MM + kd * M + l] * par[l];
-    }
-    return a0 + a1;
-  };
-  if (!band_only) {
-    double *cur = cb, *nxt = cb + M2;
-    const int sg0 = tid / M;  // valid for tid < M2
-    for (int lev = 0; lev < g.ld; lev++) {
-      const size_t r = (size_t)Rr >> (g.ld - lev);
-      const bool valid = r < ((size_t)4 << lev) - 2;
-      double v = 0;
-      if (tid < M2 && valid) v = l2l((int)(r & 1), cur + sg0 * M) + mres[lev * M2 + tid];
-      if (tid < M2) nxt[tid] = v;
-      __syncthreads();
-      double *t = cur;
-      cur = nxt;
-      nxt = t;
-    }
-    if (tid < M2) {
-      const bool valid = (size_t)Rr < ((size_t)4 << g.ld) - 2;
-      double v = 0;
-      if (valid) v = (g.ld > 0 ? l2l(Rr & 1, cur + sg0 * M) : 0.0) + mres[g.ld * M2 + tid];
-      cres[tid] = v;
-    }
-    __syncthreads();
-    const int ngrp = NT / M2;
-    for (int lam = 1; lam <= g.gd; lam++) {
-      const int n = 1 << lam, lev = g.ld + lam;
-      const int b0 = n - 1, bp = (n >> 1) - 1;
-      if (tid < ngrp * M2) {
-        const int oo = tid % M2, sg = oo / M;
-        for (int ci = tid / M2; ci < n; ci += ngrp) {
-          const size_t r = ((size_t)Rr << lam) + ci;
-          double v = 0;
-          if (r < ((size_t)4 << lev) - 2)
-            v = l2l((int)(r & 1), cres + (bp + (ci >> 1)) * M2 + sg * M) +
-                mres[(g.ld + b0 + ci) * M2 + oo];
-          cres[(b0 + ci) * M2 + oo] = v;
-        }
-      }
-      __syncthreads();
-    }
-  }
+  // ---- phase 2: local expansions of the subtree
+  if (!band_only)
+    subtree_locals<MT, false>(g, Rr, tid, NT, mres, qp, cres, qr, d0r, d1r, Qg, Dg, nullptr);
 
   // ---- phase 3: band + leaf evaluation.  Task (chunk, group gam): rows
   // gam*4.. of parity 0, then rows (ngrp-1-gam)*4.. of parity 1, so every
@@ -755,7 +929,7 @@ __global__ void __launch_bounds__(kDownNT) k_down(
         for (int rho = 0; rho < kR; rho++)
           acc[rho] = kSqrtPi * xsg[sk4(rl0 + rho)] * aw[sk8(2 * (rl0 + rho) + sg)];
       }
-      band_dispatch<DIR, PRE>(sg, aligned, xsg, aw, hd, rl0, len, acc);
+      band_dispatch<DIR, PRE>(sg, aligned, xsg, aw, htab, rl0, len, acc);
       // far field: local expansion of leaf chunk beta at the rows' nodes
       double far[kR] = {0, 0, 0, 0};
       if (!band_only && beta < nvalid) {
@@ -812,6 +986,421 @@ __global__ void __launch_bounds__(kDownNT) k_down(
 }
 
 // ---------------------------------------------------------------------------
+// k_down8: the downward pass for leaf sizes s = 16, 32, 64.  grid
+// (2^(ld+2), lines), 64 nw threads with nw = s/16 waves, G = 2^gd <= 16 leaf
+// chunks per work-group.
+//
+// Near field in 8-row register blocks.  Rows of parity sg, chunk g and 8-row
+// group gam = 2p + del run over column offsets t in [0, len), len = 2s - 8 gam
+// (shorter in the last chunk).  Pair p's columns are cut at ts = s - 8p into
+// segment A = [0, ts) and B = [ts, len); wave w runs segment A of pair w and
+// segment B of pair nw-1-w, so every wave does 2s - 8(nw-1) columns (8 fewer
+// for del = 1).  Lane = g + 16 sg + 32 del: all 64 lanes of a wave sit at the
+// same t, so the Toeplitz factor Lambda(t - rho) (dn for C2L) is one uniform
+// (scalar) load per column, and the LDS layouts make every lane address of an
+// x or Lambda read distinct modulo 32 eight-byte slots:
+//   x, per parity, half index h from the work-group start: h + h/s (one pad
+//     per chunk; a trip of 8 columns never straddles one), parity 1 at +nxs
+//     with nxs = 16 mod 32;
+//   Lambda, per chunk g, local index q = 2(r - g s) + sg + t + rho
+//     (< 3s + 16): g LW + q with LW = 2 mod 32.
+// Far field: the subtree's local expansions evaluated at the leaf nodes on
+// v_mfma_f64_16x16x4f64, F_par (s x G) = T_par (s x M) C_par (M x G), added to
+// the band sums staged in LDS; then scaling and one coalesced store.
+constexpr int kD8MaxG = 16;
+struct Down8Lds {
+  int xs, nxs, lw, LW, hd, dl, cres, mres, qp, total;
+};
+__host__ __device__ inline Down8Lds down8_lds(const Geo &g) {
+  const int G = 1 << g.gd, s = g.s, M2 = 2 * g.M;
+  Down8Lds d;
+  const int nh = (G + 1) * s + 8;  // half indices per parity, one trip of overrun
+  const int nhp = nh + nh / s + 1;
+  d.xs = 0;
+  d.nxs = nhp + (48 - nhp % 32) % 32;  // = 16 mod 32
+  d.LW = 3 * s + 16;
+  d.LW += (34 - d.LW % 32) % 32;  // = 2 mod 32
+  d.lw = d.xs + 2 * d.nxs;
+  d.hd = d.lw + G * d.LW;  // band head, 2s + 16
+  d.dl = d.hd + 2 * s + 16;  // D(0), D(1) with rows padded to M + 1
+  d.cres = d.dl + g.M * M2 + M2;
+  d.mres = d.cres + (2 * G - 1) * M2;
+  d.qp = d.mres + (g.ld + 2 * G - 1) * M2;
+  d.total = d.qp + std::max(g.ld, 1) * M2;
+  return d;  // the output staging (2 G s doubles) reuses the x windows
+}
+size_t down8_shared_bytes(const Geo &g) { return 8 * (size_t)down8_lds(g).total; }
+bool down8_eligible(const Geo &g) { return g.s == 16 || g.s == 32 || g.s == 64; }
+
+// One band segment of 8 rows: columns t in [t0, t1) (t0 = 0 or a multiple of
+// 8 shared by the wave), software-pipelined half a trip (4 columns) ahead.
+// xsg: the lane's parity window; h0: half index of the first row from the
+// work-group start; pw: the lane's Lambda window at q = t = 0 of row 0;
+// head: the zero-padded band head in LDS (read at wave-uniform addresses:
+// keeping it out of scalar memory leaves LDS reads the only users of the
+// lgkm counter, so the half-trip prefetch is not drained by scalar waits).
+// RAGGED: columns
+// t >= len are masked (the last chunk, where len is not a multiple of 8).
+template <int DIR, int PRE, bool RAGGED>
+__device__ __forceinline__ void band_seg8(const double *__restrict__ xsg,
+                                          const double *__restrict__ pw,
+                                          const double *__restrict__ head, int h0, int sh,
+                                          int t0, int t1, int len, double acc[8]) {
+  const int s = 1 << sh;
+  const int ntrip = t1 > t0 ? (t1 - t0 + 7) >> 3 : 0;
+  if (ntrip == 0) return;
+  // ring state at t0: heads Lambda(t0 - 8 + k), refills p(t0 + k)
+  double hr[8], pr[8];
+  const double *pp = pw + t0;
+#pragma unroll
+  for (int k = 0; k < 8; k++) {
+    hr[k] = t0 >= 8 ? head[t0 - 8 + k] : 0.0;
+    pr[k] = pp[k];
+  }
+  // the trip being fetched: x pointer and position inside its chunk
+  int hq = h0 + t0;
+  const double *xq = xsg + hq + (hq >> sh);
+  int mq = hq & (s - 1);
+  const double *hp = head + t0;
+  int tt = t0;
+  double XA[4], PA[4], HA[4], XB[4], PB[4], HB[4];
+  // half H (0: columns 0-3, 1: 4-7) of the current trip
+#define FETCH(X, P, H, HALF)                                         \
+  _Pragma("unroll") for (int k_ = 0; k_ < 4; k_++) {                 \
+    X[k_] = xq[4 * (HALF) + k_];                                     \
+    if (RAGGED) X[k_] = tt + 4 * (HALF) + k_ < len ? X[k_] : 0.0;    \
+    P[k_] = pp[8 + 4 * (HALF) + k_];                                 \
+    H[k_] = hp[4 * (HALF) + k_];                                     \
+  }
+#define NEXT_TRIP                   \
+  {                                 \
+    const int pad_ = mq + 8 == s;   \
+    mq = pad_ ? 0 : mq + 8;         \
+    xq += 8 + pad_;                 \
+    pp += 8;                        \
+    hp += 8;                        \
+    tt += 8;                        \
+  }
+#define TERM(hv, pv, xv, a)          \
+  if (DIR == 0)                      \
+    a += ((hv) * (pv)) * (xv);       \
+  else if (PRE)                      \
+    a -= ((hv) * (pv)) * (xv);       \
+  else                               \
+    a -= ((hv) * (xv)) * (pv);
+  // column t (ring slot K = t mod 8): head Lambda(t) enters slot K, row rho
+  // uses Lambda(t - rho) and p(t + rho), then p(t + 8) replaces p(t)
+#define STEP(K, X, P, H, J)                                \
+  {                                                        \
+    hr[K] = H[J];                                          \
+    const double x_ = X[J];                                \
+    TERM(hr[(K) & 7], pr[(K) & 7], x_, acc[0])             \
+    TERM(hr[((K) + 7) & 7], pr[((K) + 1) & 7], x_, acc[1]) \
+    TERM(hr[((K) + 6) & 7], pr[((K) + 2) & 7], x_, acc[2]) \
+    TERM(hr[((K) + 5) & 7], pr[((K) + 3) & 7], x_, acc[3]) \
+    TERM(hr[((K) + 4) & 7], pr[((K) + 4) & 7], x_, acc[4]) \
+    TERM(hr[((K) + 3) & 7], pr[((K) + 5) & 7], x_, acc[5]) \
+    TERM(hr[((K) + 2) & 7], pr[((K) + 6) & 7], x_, acc[6]) \
+    TERM(hr[((K) + 1) & 7], pr[((K) + 7) & 7], x_, acc[7]) \
+    pr[K] = P[J];                                          \
+  }
+  FETCH(XA, PA, HA, 0)
+  for (int b = 0; b < ntrip; b++) {
+    FETCH(XB, PB, HB, 1)
+    STEP(0, XA, PA, HA, 0) STEP(1, XA, PA, HA, 1) STEP(2, XA, PA, HA, 2) STEP(3, XA, PA, HA, 3)
+    NEXT_TRIP
+    FETCH(XA, PA, HA, 0)
+    STEP(4, XB, PB, HB, 0) STEP(5, XB, PB, HB, 1) STEP(6, XB, PB, HB, 2) STEP(7, XB, PB, HB, 3)
+  }
+#undef STEP
+#undef TERM
+#undef NEXT_TRIP
+#undef FETCH
+}
+
+// Modes: K8_FULL band and far field; K8_BAND the near field alone, scaled
+// (directM).
+enum { K8_FULL = 0, K8_BAND = 1 };
+
+template <int DIR, int PRE, int MT>
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) k_down8(
+    const double *__restrict__ in, double *__restrict__ out, LineMap lm, Geo g, int mode,
+    const double *__restrict__ wsall, size_t wsstride, const double *__restrict__ Dg,
+    const double *__restrict__ Qg, const double *__restrict__ Tg, const double *__restrict__ tab,
+    const double *__restrict__ head) {
+  extern __shared__ double sm[];
+  const int M = MT > 0 ? MT : g.M;
+  const int s = g.s, N = g.N, M2 = 2 * M, MM = M * M;
+  const int G = 1 << g.gd, sh = 31 - __builtin_clz(s);
+  const int nw = s >> 4, NT = 64 * nw;
+  const int Rr = blockIdx.x;
+  const int tid = threadIdx.x, lane = tid & 63;
+  // the wave index is wave-uniform; saying so lets the band's column offsets
+  // and head loads live in scalar registers
+  const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
+  const bool do_band = true, do_far = mode == K8_FULL;
+  const size_t row0 = (size_t)Rr * G * 2 * s;
+  if (row0 >= (size_t)N) return;  // the whole work-group
+  const int nr = (int)std::min((size_t)G * 2 * s, (size_t)N - row0);
+  const size_t hc0 = (size_t)Rr * G * s;  // first half index of the work-group
+  const size_t off = line_off(lm, blockIdx.y);
+  const double *src = in + off;
+  double *dst = out + off;
+  const double *m2l = wsall + blockIdx.y * wsstride + ws_layout(g).m2l;
+  const Down8Lds ly = down8_lds(g);
+  double *xs = sm + ly.xs, *lw = sm + ly.lw, *cres = sm + ly.cres;
+  double *mres = sm + ly.mres, *qp = sm + ly.qp, *hd = sm + ly.hd, *dl = sm + ly.dl;
+  // output staging after the band: segment A sums in ob (the x windows),
+  // segment B sums in ob2 (the Lambda windows); slot of output e: e + e / 2s
+  // (one pad per chunk spreads the lanes of every writer over distinct banks)
+  double *ob = xs, *ob2 = lw;
+  auto slot = [&](int e) { return e + (e >> (sh + 1)); };
+
+  // ---- phase 0: every global load of the kernel issued up front (clamped
+  // addresses: one straight-line batch).  The tree operands (M2L vectors, the
+  // chain row of Q, D) are needed first; the band windows (x, the Lambda span,
+  // the head) stay in flight through the tree pass, whose barriers are
+  // LDS-only.
+  const int nm = do_far ? (g.ld + 2 * G - 1) * M2 : 0;
+  constexpr int QM = 6;
+  double vm[QM];
+  bool okm[QM];
+  const int nq = do_far ? g.ld * M : 0;
+  double qr[MT > 0 ? MT : 1];
+  constexpr int QD = MT > 0 ? (2 * MT * MT + 63) / 64 : 1;  // D(0), D(1) for the LDS copy
+  double vd[QD];
+  if (do_far) {
+#pragma unroll
+    for (int q = 0; q < QM; q++) {
+      const int e = q * NT + tid;
+      const size_t a = m2l_elem(g, Rr, e < nm ? e : 0, M2, okm[q]);
+      okm[q] = okm[q] && e < nm;
+      vm[q] = m2l[okm[q] ? a : 0];
+    }
+    if (MT > 0 && nq > 0) {
+      const double *qrow = qrow_ptr(Qg, g, Rr, tid < nq ? tid : 0, M);
+#pragma unroll
+      for (int l = 0; l < (MT > 0 ? MT : 1); l++) qr[l] = qrow[l];
+    }
+#pragma unroll
+    for (int q = 0; q < QD; q++) {
+      const int e = q * NT + tid;
+      vd[q] = Dg[e < 2 * MM ? e : 0];
+    }
+  }
+  // keep the tree operands ahead of the windows in the memory pipeline: the
+  // stores below then wait for them alone
+  __builtin_amdgcn_sched_barrier(0);
+  // x: input elements row0 + e, e < nX (both parities interleaved)
+  const int nX = do_band ? 2 * ((G + 1) * s + 8) : 0;
+  constexpr int QX = 9;  // ceil(nX / NT) for G = 16
+  double vx[QX];
+  // Lambda: the contiguous span 2 hc0 + e, e < 2Gs + s + 16; element e lies
+  // in chunk window e / 2s and, when e mod 2s < s + 16, also in the one before
+  const int QL = 3 * s + 16, nLs = do_band ? 2 * G * s + s + 16 : 0;
+  constexpr int QLS = 9;  // ceil(nLs / NT) for G = 16
+  double vl[QLS];
+  const int nhd = 2 * s + 16;  // <= NT
+  double vh = 0.0;
+  if (do_band) {
+#pragma unroll
+    for (int q = 0; q < QX; q++)
+      vx[q] = src[std::min(row0 + q * NT + tid, (size_t)N - 1) * lm.es];
+#pragma unroll
+    for (int q = 0; q < QLS; q++)
+      vl[q] = tab[std::min(2 * hc0 + q * NT + tid, (size_t)N - 1)];
+    vh = head[tid < nhd ? tid : 0];
+  }
+  __builtin_amdgcn_sched_barrier(0);
+  if (do_far) {
+#pragma unroll
+    for (int q = 0; q < QM; q++) {
+      const int e = q * NT + tid;
+      if (e < nm) mres[e] = okm[q] ? vm[q] : 0.0;
+    }
+  }
+  for (int e = QM * NT + tid; e < nm; e += NT) {
+    bool ok;
+    const size_t a = m2l_elem(g, Rr, e, M2, ok);
+    mres[e] = ok ? m2l[a] : 0.0;
+  }
+  if (do_far) {
+#pragma unroll
+    for (int q = 0; q < QD; q++) {
+      const int e = q * NT + tid;
+      if (MT > 0 && e < 2 * MM) dl[(e / M) * (M + 1) + e % M] = vd[q];
+    }
+    for (int e = (MT > 0 ? QD * NT : 0) + tid; e < 2 * MM; e += NT)
+      dl[(e / M) * (M + 1) + e % M] = Dg[e];
+  }
+  lds_sync();
+
+  // ---- phase 2: local expansions of the subtree
+  if (do_far)
+    subtree_locals<MT, true>(g, Rr, tid, NT, mres, qp, cres, qr, nullptr, nullptr, Qg, Dg, dl);
+
+  // A operand (T) of this wave's two far-field tiles, in flight during the band
+  const int ntm = s >> 4, ksteps = (M + 3) >> 2;
+  constexpr int QK = MT > 0 ? (MT + 3) / 4 : 1;
+  double ta[2][QK];
+#pragma unroll
+  for (int j = 0; j < 2; j++) {
+    const int tile = wave + j * nw, sgf = tile / ntm, rt = tile - sgf * ntm;
+    const int m = rt * 16 + (lane & 15);
+#pragma unroll
+    for (int kk = 0; kk < QK; kk++) {
+      const int k = kk * 4 + (lane >> 4);
+      ta[j][kk] = (MT > 0 && do_far && k < M) ? Tg[((size_t)sgf * s + m) * M + k] : 0.0;
+    }
+  }
+
+  if (do_band) {
+    // ---- the band windows
+#pragma unroll
+    for (int q = 0; q < QX; q++) {
+      const int e = q * NT + tid;
+      if (e < nX) {
+        const size_t i = row0 + e;
+        const double u = i < (size_t)N ? vx[q] : 0.0;
+        const int h = e >> 1;
+        xs[(e & 1) * ly.nxs + h + (h >> sh)] = (DIR == 1 && i > 0) ? u * (double)i : u;  // v_i = i u_i
+      }
+    }
+    for (int e = QX * NT + tid; e < nX; e += NT) {
+      const size_t i = row0 + e;
+      double u = i < (size_t)N ? src[i * lm.es] : 0.0;
+      if (DIR == 1 && i > 0) u *= (double)i;
+      const int h = e >> 1;
+      xs[(e & 1) * ly.nxs + h + (h >> sh)] = u;
+    }
+    auto put_lambda = [&](int e, double v) {
+      const int c = e >> (sh + 1), q = e - (c << (sh + 1));
+      if (2 * hc0 + e >= (size_t)N) v = 0.0;
+      if (c < G) lw[c * ly.LW + q] = v;
+      if (c > 0 && q + 2 * s < QL) lw[(c - 1) * ly.LW + q + 2 * s] = v;
+    };
+#pragma unroll
+    for (int q = 0; q < QLS; q++) {
+      const int e = q * NT + tid;
+      if (e < nLs) put_lambda(e, vl[q]);
+    }
+    for (int e = QLS * NT + tid; e < nLs; e += NT)
+      put_lambda(e, tab[std::min(2 * hc0 + e, (size_t)N - 1)]);
+    if (tid < nhd) hd[tid] = vh;
+    lds_sync();
+
+    // ---- phase 3: near field.  Lane (chunk gl, parity sg, del); wave:
+    // segment A of pair wave, segment B of pair nw-1-wave
+    const int gl = lane & 15, sg = (lane >> 4) & 1, del = lane >> 5;
+    const int cnt = (N - sg + 1) / 2;  // elements of parity sg
+    const size_t beta = (size_t)Rr * G + gl;
+    const size_t nLm1 = (size_t)g.nL - 1;
+    const size_t cend = beta < nLm1 ? (beta + 2) * s : (size_t)cnt;
+    const bool ragged = (size_t)(Rr + 1) * G > nLm1;  // work-group holds the last chunk
+    const double *xsg = xs + sg * ly.nxs;
+    double accA[8], accB[8];
+    int rA = -1, rB = -1;  // first row (half index) of each segment, -1: none
+    {
+      const int p = wave, gam = 2 * p + del, ts = s - 8 * p;
+      const size_t r0 = beta * s + 8 * gam;
+      const bool live = gl < G && r0 < (size_t)cnt;
+      const int len = live ? (int)(cend - r0) : 0;
+      const int h0 = (int)(r0 - hc0);
+      const double *pw = lw + gl * ly.LW + 16 * gam + sg;
+#pragma unroll
+      for (int k = 0; k < 8; k++) accA[k] = 0.0;
+      if (live) {
+        rA = h0;
+        if (DIR == 1) {  // C2L diagonal: sqrt(pi) an_i v_i (src/C/leg2cheb.c:416)
+          const double *xr = xsg + h0 + (h0 >> sh);
+#pragma unroll
+          for (int k = 0; k < 8; k++) accA[k] = kSqrtPi * xr[k] * pw[2 * k];
+        }
+      }
+      if (ragged)
+        band_seg8<DIR, PRE, true>(xsg, pw, hd, h0, sh, 0, std::min(ts, len), len, accA);
+      else
+        band_seg8<DIR, PRE, false>(xsg, pw, hd, h0, sh, 0, ts, len, accA);
+    }
+    {
+      const int p = nw - 1 - wave, gam = 2 * p + del, ts = s - 8 * p;
+      const size_t r0 = beta * s + 8 * gam;
+      const bool live = gl < G && r0 < (size_t)cnt;
+      const int len = live ? (int)(cend - r0) : 0;
+      const int h0 = (int)(r0 - hc0);
+      const double *pw = lw + gl * ly.LW + 16 * gam + sg;
+#pragma unroll
+      for (int k = 0; k < 8; k++) accB[k] = 0.0;
+      if (live) rB = h0;
+      if (ragged)
+        band_seg8<DIR, PRE, true>(xsg, pw, hd, h0, sh, ts, len, len, accB);
+      else
+        band_seg8<DIR, PRE, false>(xsg, pw, hd, h0, sh, ts, len, len, accB);
+    }
+    lds_sync();  // windows dead: the staging takes their place
+
+    // ---- phase 4: segment A sums to ob, segment B sums to ob2 (each row
+    // once in each)
+    auto stage = [&](int rl, const double *acc, double *dstg) {
+      if (rl < 0) return;
+#pragma unroll
+      for (int k = 0; k < 8; k++)
+        if (hc0 + rl + k < (size_t)cnt) dstg[slot(2 * (rl + k) + sg)] = acc[k];
+    };
+    stage(rA, accA, ob);
+    stage(rB, accB, ob2);
+    lds_sync();
+  }
+  if (do_far) {
+    // F_par^T (G x s) = C_par^T (G x M) T_par^T (M x s), tile (parity sgf,
+    // rows 16 rt..): A[n][k] = c_k of leaf n, B[k][m] = T_par[16 rt + m][k];
+    // lane l then holds leaf l/16 + 4v, row 16 rt + l%16; added into ob.
+    const int n = lane & 15;
+#pragma unroll
+    for (int j = 0; j < 2; j++) {
+      const int tile = wave + j * nw, sgf = tile / ntm, rt = tile - sgf * ntm;
+      d4 D = {0, 0, 0, 0};
+      if (MT > 0) {
+        double av[QK];
+#pragma unroll
+        for (int kk = 0; kk < QK; kk++) {
+          const int k = kk * 4 + (lane >> 4);
+          av[kk] = (n < G && k < M) ? cres[(G - 1 + n) * M2 + sgf * M + k] : 0.0;
+        }
+#pragma unroll
+        for (int kk = 0; kk < QK; kk++) D = __builtin_amdgcn_mfma_f64_16x16x4f64(av[kk], ta[j][kk], D, 0, 0, 0);
+      } else {
+        for (int kk = 0; kk < ksteps; kk++) {
+          const int k = kk * 4 + (lane >> 4);
+          const double bv = k < M ? Tg[((size_t)sgf * s + rt * 16 + n) * M + k] : 0.0;
+          const double av = (n < G && k < M) ? cres[(G - 1 + n) * M2 + sgf * M + k] : 0.0;
+          D = __builtin_amdgcn_mfma_f64_16x16x4f64(av, bv, D, 0, 0, 0);
+        }
+      }
+#pragma unroll
+      for (int v = 0; v < 4; v++) {
+        const int leaf = (lane >> 4) + 4 * v, m = rt * 16 + n;
+        const int e = 2 * (leaf * s + m) + sgf;
+        if (leaf < G && e < nr) ob[slot(e)] += D[v];
+      }
+    }
+    lds_sync();
+  }
+  // ---- store: the scaled sum
+  for (int e = tid; e < nr; e += NT) {
+    const size_t i = row0 + e;
+    double v = ob[slot(e)] + ob2[slot(e)];
+    if (DIR == 0) {
+      v *= k2Pi;
+      if (i == 0) v *= 0.5;
+    } else {
+      v *= (double)i + 0.5;
+    }
+    dst[i * lm.es] = v;
+  }
+}
+
+// ---------------------------------------------------------------------------
 // direct O(N^2) sums, one row per thread (src/C/leg2cheb.c:152-234)
 __global__ void __launch_bounds__(256) k_direct(const double *__restrict__ in,
                                                 double *__restrict__ out, LineMap lm, int N,
@@ -855,10 +1444,35 @@ static int ensure_ws(fmm_plan_s *p, size_t nlines, double **w) {
   return 0;
 }
 
+// k_down8 launch in the given mode on stream st
 template <int MT>
-static void launch_fmm_m(fmm_plan_s *p, const double *d_in, double *d_out, size_t dir,
-                         const LineMap &lm, size_t nlines, hipStream_t st, int band_only,
-                         double *ws) {
+static void launch_down8(fmm_plan_s *p, const double *d_in, double *d_out, size_t dir,
+                         const LineMap &lm, size_t nlines, hipStream_t st, int mode, double *ws) {
+  const Geo &g = p->geo;
+  const size_t wsstride = ws_doubles(g);
+  dim3 gd((unsigned)((size_t)4 << g.ld), (unsigned)nlines);
+  const size_t shm = down_shared_bytes(g);
+  const int nt = 64 * (g.s >> 4);
+  const double *tab = dir == L2C ? p->t.a : p->t.an;
+  const double *htab = p->t.head[dir];
+  prof_begin(p->prof, K_DOWN_BAND, st);
+  if (dir == L2C)
+    k_down8<0, 0, MT><<<gd, nt, shm, st>>>(d_in, d_out, lm, g, mode, ws, wsstride, p->t.D,
+                                           p->t.Q, p->t.T, tab, htab);
+  else if (p->precompute)
+    k_down8<1, 1, MT><<<gd, nt, shm, st>>>(d_in, d_out, lm, g, mode, ws, wsstride, p->t.D,
+                                           p->t.Q, p->t.T, tab, htab);
+  else
+    k_down8<1, 0, MT><<<gd, nt, shm, st>>>(d_in, d_out, lm, g, mode, ws, wsstride, p->t.D,
+                                           p->t.Q, p->t.T, tab, htab);
+  prof_end(p->prof, st);
+}
+
+// One execute (or a batch of lines): upward pass, M2L, downward pass, all on st.
+template <int MT>
+static int launch_fmm_m(fmm_plan_s *p, const double *d_in, double *d_out, size_t dir,
+                        const LineMap &lm, size_t nlines, hipStream_t st, int band_only,
+                        double *ws) {
   const Geo &g = p->geo;
   const size_t wsstride = ws_doubles(g);
   if (!band_only) {
@@ -874,36 +1488,41 @@ static void launch_fmm_m(fmm_plan_s *p, const double *d_in, double *d_out, size_
     k_m2l<MT><<<gm, kM2lNT, m2l_shared_bytes(g), st>>>(g, p->t.A[dir], ws, wsstride, bpw);
     prof_end(p->prof, st);
   }
+  if (g.fast) {
+    launch_down8<MT>(p, d_in, d_out, dir, lm, nlines, st, band_only ? K8_BAND : K8_FULL, ws);
+    return 0;
+  }
   dim3 gd((unsigned)((size_t)4 << g.ld), (unsigned)nlines);
   const size_t shm = down_shared_bytes(g);
   const double *tab = dir == L2C ? p->t.a : p->t.an;
-  const double *htab = dir == L2C ? p->t.a : p->t.dn;
+  const double *htab = p->t.head[dir];
   prof_begin(p->prof, K_DOWN_BAND, st);
   if (dir == L2C)
     k_down<0, 0, MT><<<gd, kDownNT, shm, st>>>(d_in, d_out, lm, g, band_only, ws, wsstride,
-                                               p->t.D, p->t.T, tab, htab);
+                                               p->t.D, p->t.Q, p->t.T, tab, htab);
   else if (p->precompute)
     k_down<1, 1, MT><<<gd, kDownNT, shm, st>>>(d_in, d_out, lm, g, band_only, ws, wsstride,
-                                               p->t.D, p->t.T, tab, htab);
+                                               p->t.D, p->t.Q, p->t.T, tab, htab);
   else
     k_down<1, 0, MT><<<gd, kDownNT, shm, st>>>(d_in, d_out, lm, g, band_only, ws, wsstride,
-                                               p->t.D, p->t.T, tab, htab);
+                                               p->t.D, p->t.Q, p->t.T, tab, htab);
   prof_end(p->prof, st);
+  return 0;
 }
 
 static int launch_fmm(fmm_plan_s *p, const double *d_in, double *d_out, size_t dir,
                       const LineMap &lm, size_t nlines, hipStream_t st, int band_only) {
   double *ws = nullptr;
   if (ensure_ws(p, nlines, &ws)) return -1;
-  if (p->geo.M == 18)
-    launch_fmm_m<18>(p, d_in, d_out, dir, lm, nlines, st, band_only, ws);
-  else
-    launch_fmm_m<0>(p, d_in, d_out, dir, lm, nlines, st, band_only, ws);
+  const int rc = p->geo.M == 18 ? launch_fmm_m<18>(p, d_in, d_out, dir, lm, nlines, st, band_only, ws)
+                                : launch_fmm_m<0>(p, d_in, d_out, dir, lm, nlines, st, band_only, ws);
+  if (rc) return rc;
   L2C_HIP(hipGetLastError(), -1);
   return 0;
 }
 int run_fmm(fmm_plan_s *p, const double *d_in, double *d_out, size_t dir, size_t stride,
             hipStream_t st, int band_only) {
+  prof_step(p->prof);
   LineMap lm{1, 0, 0, stride};
   return launch_fmm(p, d_in, d_out, dir, lm, 1, st, band_only);
 }
@@ -911,6 +1530,7 @@ int run_fmm(fmm_plan_s *p, const double *d_in, double *d_out, size_t dir, size_t
 int run_direct(const direct_plan_s *dp, const double *d_in, double *d_out, size_t dir,
                size_t stride, hipStream_t st, size_t precompute) {
   (void)precompute;
+  prof_step(dp->prof);
   LineMap lm{1, 0, 0, stride};
   const int N = (int)dp->N;
   prof_begin(dp->prof, K_DIRECT, st);
@@ -923,6 +1543,7 @@ int run_direct(const direct_plan_s *dp, const double *d_in, double *d_out, size_
 
 int run_lines(fmm_plan_s *p, const double *d_in, double *d_out, size_t dir, size_t nlines,
               size_t n0, size_t ls0, size_t ls1, size_t es, hipStream_t st) {
+  prof_step(p->prof);
   LineMap lm{n0 ? n0 : 1, ls0, ls1, es};
   if (p->L == 0) {
     const direct_plan_s *dp = p->dplan;

@@ -98,6 +98,7 @@ struct Geo {
   int gu, lu;  // upward work-group: 2^gu leaf chunks; cut level lu = L-1-gu
   int gd, ld;  // downward work-group: 2^gd leaf chunks; cut level ld = L-1-gd
   int nodal;   // 1: Lagrange basis (lagrange=1), 0: Chebyshev basis
+  int fast;    // 1: downward pass by k_down8 (s = 16, 32 or 64)
 };
 
 struct DevTables {
@@ -109,13 +110,19 @@ struct DevTables {
   double *U = nullptr;   // 2 x M x M  upward (child q -> parent), row-major
   double *D = nullptr;   // 2 x M x M  downward (parent -> child p), row-major
   double *P = nullptr;   // composite upward operators for the top levels
+  double *Q = nullptr;   // composite downward operators for the ancestor chain
+  // band head per direction: Lambda(k) (L2C) or dn(k) (C2L) for k < min(2s,
+  // table length), zero up to 2s + 16
+  double *head[2] = {nullptr, nullptr};
   double *Asl[2] = {nullptr, nullptr};  // split-precision M2L slices
 };
 
 // Per-launch HIP-event timing of the execute kernels (bench support).
 enum KernelId { K_UP_SUBTREE = 0, K_M2L = 1, K_DOWN_BAND = 2, K_DIRECT = 3, K_COUNT = 8 };
 struct Prof {
-  int on = 0;
+  int on = 0;       // timing period in calls (0: off)
+  size_t calls = 0;  // calls since enabled
+  int cur = 0;      // the current call is timed
   struct Rec {
     int id;
     hipEvent_t a, b;
@@ -126,6 +133,7 @@ struct Prof {
   double ms[K_COUNT] = {0};
   size_t cnt[K_COUNT] = {0};
 };
+void prof_step(Prof *pf);  // at the start of each call
 void prof_begin(Prof *pf, int id, hipStream_t st);
 void prof_end(Prof *pf, hipStream_t st);
 }  // namespace l2c
@@ -174,7 +182,8 @@ int run_direct(const direct_plan_s *dp, const double *d_in, double *d_out, size_
 int run_lines(fmm_plan_s *p, const double *d_in, double *d_out, size_t dir, size_t nlines,
               size_t n0, size_t ls0, size_t ls1, size_t es, hipStream_t st);
 size_t flops_fmm(const fmm_plan_s *p, size_t dir);
-size_t down_shared_bytes(const Geo &g);
+size_t down_shared_bytes(const Geo &g);  // of the downward kernel g selects
+bool down8_eligible(const Geo &g);
 size_t up_shared_bytes(const Geo &g);
 size_t ws_doubles(const Geo &g);  // execute workspace per line
 size_t flops_direct(size_t N, size_t dir, size_t precompute);

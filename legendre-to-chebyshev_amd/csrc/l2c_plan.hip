@@ -325,6 +325,8 @@ void destroy_direct(direct_plan_s *dp) {
 
 static void destroy_fmm(fmm_plan_s *p) {
   if (!p) return;
+  // nothing may still read the tables or the workspace
+  if (p->stream) (void)hipStreamSynchronize(p->stream);
   dfree(p->t.a);
   dfree(p->t.an);
   dfree(p->t.dn);
@@ -336,6 +338,9 @@ static void destroy_fmm(fmm_plan_s *p) {
   dfree(p->t.U);
   dfree(p->t.D);
   dfree(p->t.P);
+  dfree(p->t.Q);
+  dfree(p->t.head[0]);
+  dfree(p->t.head[1]);
   dfree(p->d_w);
   dfree(p->d_wl);
   dfree(p->d_in);
@@ -356,12 +361,16 @@ static void destroy_fmm(fmm_plan_s *p) {
 // until the work-group's LDS fits the budget.
 static bool choose_cuts(Geo &g) {
   const int Lm1 = g.L - 1;
-  int gu = 4, gd = 3;
+  // k_down8 (s = 16, 32, 64; L2C_DOWN8=0 selects k_down): 16 leaf chunks and
+  // up to 80 KiB of LDS (two work-groups per CU); k_down: 8 and 64 KiB
+  const char *e8 = getenv("L2C_DOWN8");
+  g.fast = down8_eligible(g) && !(e8 && atoi(e8) == 0) ? 1 : 0;
+  int gu = 4, gd = g.fast ? 4 : 3;
   if (const char *e = getenv("L2C_GU")) gu = atoi(e);
   if (const char *e = getenv("L2C_GD")) gd = atoi(e);
-  const size_t budget = 64 * 1024;
+  const size_t budget = 64 * 1024, dbudget = (g.fast ? 80 : 64) * 1024;
   g.gu = std::max(0, std::min(std::min(gu, 4), Lm1));  // k_up: at most 16 leaf chunks
-  g.gd = std::max(0, std::min(gd, Lm1));
+  g.gd = std::max(0, std::min(std::min(gd, g.fast ? 4 : Lm1), Lm1));
   for (;;) {
     g.lu = Lm1 - g.gu;
     g.ld = Lm1 - g.gd;
@@ -369,13 +378,13 @@ static bool choose_cuts(Geo &g) {
     if (up_shared_bytes(g) > budget && g.gu > 0) {
       g.gu--;
       shrink = true;
-    } else if (down_shared_bytes(g) > budget && g.gd > 0) {
+    } else if (down_shared_bytes(g) > dbudget && g.gd > 0) {
       g.gd--;
       shrink = true;
     }
     if (!shrink) break;
   }
-  return up_shared_bytes(g) <= budget && down_shared_bytes(g) <= budget;
+  return up_shared_bytes(g) <= budget && down_shared_bytes(g) <= dbudget;
 }
 
 fmm_plan_s *create_fmm_impl(size_t N, size_t maxs, size_t M, size_t direction, size_t lagrange,
@@ -399,7 +408,7 @@ fmm_plan_s *create_fmm_impl(size_t N, size_t maxs, size_t M, size_t direction, s
   (void)hipGetDevice(&p->device);
   if (hipStreamCreateWithFlags(&p->stream, hipStreamNonBlocking) != hipSuccess) {
     set_error("hipStreamCreate failed");
-    delete p;
+    destroy_fmm(p);
     return nullptr;
   }
   // geometry: src/C/leg2cheb.c:860-871
@@ -458,6 +467,19 @@ fmm_plan_s *create_fmm_impl(size_t N, size_t maxs, size_t M, size_t direction, s
   if (build_lambda_tables(N, c2l, &p->t.a, &p->t.an, &p->t.dn, st, acct)) {
     destroy_fmm(p);
     return nullptr;
+  }
+  // band heads, read as uniform (scalar) loads by the band loop
+  for (int d = 0; d < 2; d++) {
+    if ((d == 0 && !l2c) || (d == 1 && !c2l)) continue;
+    const size_t nhd = 2 * s + 16, nh = std::min(2 * s, d == 0 ? N : (N + 1) / 2);
+    if (dmalloc(&p->t.head[d], nhd, acct)) {
+      destroy_fmm(p);
+      return nullptr;
+    }
+    L2C_HIP(hipMemsetAsync(p->t.head[d], 0, nhd * sizeof(double), st), (destroy_fmm(p), nullptr));
+    L2C_HIP(hipMemcpyAsync(p->t.head[d], d == 0 ? p->t.a : p->t.dn, nh * sizeof(double),
+                           hipMemcpyDeviceToDevice, st),
+            (destroy_fmm(p), nullptr));
   }
 
   // host tables: nodes, cosine table, T, U, D, P
@@ -550,12 +572,33 @@ fmm_plan_s *create_fmm_impl(size_t N, size_t maxs, size_t M, size_t direction, s
       }
     }
   }
+  // composite downward operators Q(k, d), k = 1..ld, d < 2^k: the L2L chain
+  // from a level-l chunk down k levels to its descendant whose low k index
+  // bits are d.  Q(1,q) = D(q), Q(k,d) = D(d & 1) Q(k-1, d >> 1).
+  const int ldq = g.ld;
+  const size_t nQ = ldq > 0 ? (((size_t)2 << ldq) - 2) : 0;
+  std::vector<double> Q(std::max<size_t>(nQ, 1) * MM);
+  for (int k = 1; k <= ldq; k++) {
+    const size_t base = ((size_t)1 << k) - 2;
+    for (size_t d = 0; d < ((size_t)1 << k); d++) {
+      double *dst = &Q[(base + d) * MM];
+      if (k == 1) {
+        std::copy(&D[d * MM], &D[d * MM] + MM, dst);
+      } else {
+        const size_t pb = ((size_t)1 << (k - 1)) - 2;
+        matmul(Mi, &D[(d & 1) * MM], &Q[(pb + (d >> 1)) * MM], dst);
+      }
+    }
+  }
 
   if (dmalloc(&p->t.T, 2 * s * M, acct) || dmalloc(&p->t.U, 2 * MM, acct) ||
-      dmalloc(&p->t.D, 2 * MM, acct) || dmalloc(&p->t.P, std::max<size_t>(nP, 1) * MM, acct)) {
+      dmalloc(&p->t.D, 2 * MM, acct) || dmalloc(&p->t.P, std::max<size_t>(nP, 1) * MM, acct) ||
+      dmalloc(&p->t.Q, std::max<size_t>(nQ, 1) * MM, acct)) {
     destroy_fmm(p);
     return nullptr;
   }
+  if (nQ)
+    L2C_HIP(hipMemcpyAsync(p->t.Q, Q.data(), nQ * MM * 8, hipMemcpyHostToDevice, st), (destroy_fmm(p), nullptr));
   L2C_HIP(hipMemcpyAsync(p->t.T, T.data(), T.size() * 8, hipMemcpyHostToDevice, st), (destroy_fmm(p), nullptr));
   L2C_HIP(hipMemcpyAsync(p->t.U, U.data(), U.size() * 8, hipMemcpyHostToDevice, st), (destroy_fmm(p), nullptr));
   L2C_HIP(hipMemcpyAsync(p->t.D, D.data(), D.size() * 8, hipMemcpyHostToDevice, st), (destroy_fmm(p), nullptr));
@@ -762,8 +805,8 @@ int l2c_synchronize(fmm_plan *p) {
   return 0;
 }
 int l2c_set_precision(fmm_plan *p, int form) {
-  if (!p || form < 0 || form > 1) {
-    set_error("l2c_set_precision: form must be 0 or 1");
+  if (!p || form != 0) {
+    set_error("l2c_set_precision: only the plain FP64 form (0) is available");
     return -1;
   }
   p->precision = form;

@@ -15,8 +15,11 @@ def tol(N):
     return 1e-14 * math.log2(max(N, 2))
 
 
-CASES = [(257, 64), (300, 32), (512, 64), (1000, 32), (1025, 64), (4096, 64), (8191, 64),
-         (8192, 32), (12345, 64), (65536, 64)]
+# leaf sizes s: 33 (257), 19 (300), 64 (512), 32 (1000), 33 (1025), 64 (4096), 16 (4096/16),
+# 64 (8191, last chunk ragged), 32 (8192), 32 (16001, ragged), 16 (4000, ragged), 49 (12345),
+# 64 (65000, ragged), 64 (65536); s = 16, 32, 64 take the k_down8 kernel
+CASES = [(257, 64), (300, 32), (512, 64), (1000, 32), (1025, 64), (4096, 64), (4096, 16),
+         (8191, 64), (8192, 32), (16001, 32), (4000, 16), (12345, 64), (65000, 64), (65536, 64)]
 
 
 @pytest.mark.parametrize("N,maxs", CASES)
@@ -72,7 +75,7 @@ def test_round_trip(amd, N, maxs):
     assert np.max(np.abs(c - u)) < 1e-10
 
 
-@pytest.mark.parametrize("N,stride", [(1000, 3), (4097, 2), (200, 5)])
+@pytest.mark.parametrize("N,stride", [(1000, 3), (4097, 2), (200, 5), (8192, 3)])
 def test_strided(amd, orc, N, stride):
     announce(f"strided N={N} stride={stride}")
     u = amd.coefficients(N, seed=11)
@@ -89,20 +92,20 @@ def test_strided(amd, orc, N, stride):
         assert np.all(out[mask] == 7.0)
 
 
+@pytest.mark.parametrize("N,maxs", [(3000, 32), (4096, 64)])
 @pytest.mark.parametrize("lagrange,precompute", [(1, 0), (0, 1), (1, 1)])
-def test_variants(amd, orc, lagrange, precompute):
-    N = 3000
-    announce(f"variant lagrange={lagrange} precompute={precompute}")
+def test_variants(amd, orc, N, maxs, lagrange, precompute):
+    announce(f"variant N={N} maxs={maxs} lagrange={lagrange} precompute={precompute}")
     u = amd.coefficients(N, seed=5)
     for d in (0, 1):
-        g = amd.FMMPlan(N, 32, 18, d, lagrange, precompute).execute(u, d)
-        o = orc.FMMPlan(N, 32, 18, d, lagrange, precompute).execute(u, d)
+        g = amd.FMMPlan(N, maxs, 18, d, lagrange, precompute).execute(u, d)
+        o = orc.FMMPlan(N, maxs, 18, d, lagrange, precompute).execute(u, d)
         assert relerr(g, o) <= tol(N)
 
 
-def test_directM(amd, orc):
-    N = 5000
-    announce("directM")
+@pytest.mark.parametrize("N", [5000, 4096])
+def test_directM(amd, orc, N):
+    announce(f"directM N={N}")
     u = amd.coefficients(N, seed=9)
     g = amd.FMMPlan(N, 64, 18, 0).directM(u)
     o = np.zeros(N)
@@ -111,11 +114,28 @@ def test_directM(amd, orc):
     assert relerr(g, o) <= tol(N)
 
 
+@pytest.mark.parametrize("N,maxs,M", [(65536, 64, 18), (8191, 64, 24), (4096, 32, 12)])
+def test_downward_kernels_agree(amd, orc, monkeypatch, N, maxs, M):
+    """The s = 16/32/64 downward kernel and the general one give the same
+    transform (L2C_DOWN8=0 selects the general kernel at plan creation)."""
+    announce(f"k_down8 vs k_down N={N} maxs={maxs} M={M}")
+    u = amd.coefficients(N, seed=21)
+    for d in (0, 1):
+        fast = amd.FMMPlan(N, maxs, M, d).execute(u, d)
+        monkeypatch.setenv("L2C_DOWN8", "0")
+        gen = amd.FMMPlan(N, maxs, M, d).execute(u, d)
+        monkeypatch.delenv("L2C_DOWN8")
+        o = orc.FMMPlan(N, maxs, M, d).execute(u, d)
+        t = tol(N) if M >= 18 else 1e-9
+        assert relerr(fast, o) <= t and relerr(gen, o) <= t
+        assert relerr(fast, gen) <= tol(N)
+
+
+@pytest.mark.parametrize("N0,N1", [(400, 402), (512, 600)])
 @pytest.mark.parametrize("axis", [0, 1, -1])
-def test_2d(amd, orc, axis):
+def test_2d(amd, orc, axis, N0, N1):
     """src/bin/l2c.c:297-393 (-d4, -d5)."""
-    N0, N1 = 400, 402
-    announce(f"2d axis={axis}")
+    announce(f"2d {N0}x{N1} axis={axis}")
     rng = np.random.default_rng(1)
     x = rng.uniform(-1, 1, (N0, N1))
     for d in (0, 1):
