@@ -26,6 +26,11 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 PKG_DIR = os.path.join(ROOT, "legendre-to-chebyshev_amd")
 HBM_PEAK_GBS = 8000.0
 METRIC = "L2C transforms/s at N=2^20 and batched N<=8192; max rel err vs FP64 oracle"
+# The reference's published L2C figure at N = 2^20: 1.6530e-02 s per transform
+# on one MacBook M3 Pro core, maxs = 32 (s = 32), M = 18
+# (src/results/figure4.sh:54).
+PUBLISHED = {"value": 60.5, "unit": "transforms/s", "source": "src/results/figure4.sh:54",
+             "hardware": "MacBook M3 Pro, 1 core", "config": "L2C N=2^20 maxs=32 (s=32) M=18"}
 
 
 def load_pkg():
@@ -167,6 +172,10 @@ def main():
                     help="rows checked against the double-double direct sum")
     ap.add_argument("--prof-every", type=int, default=16,
                     help="time the kernels of every n-th step with HIP events")
+    ap.add_argument("--compare-maxs", type=int, default=32,
+                    help="also time this maxs (the published figure's leaf size); 0 = skip")
+    ap.add_argument("--pcie-steps", type=int, default=20,
+                    help="host-pointer execute() calls timed for the PCIe-inclusive rate; 0 = skip")
     ap.add_argument("--copy-mb", type=int, default=0,
                     help="run a plain copy of this many MiB first (--pmc calibration)")
     ap.add_argument("--pmc-summary", nargs=2, metavar=("FETCH_DIR", "WRITE_DIR"),
@@ -259,6 +268,38 @@ def main():
                 "traffic": pmc_traffic(workload, dom), "algorithmic_bytes": alg,
                 "avg_launch_us": round(avg_us, 3)}
 
+    # the drop-in execute() on host arrays: staged through pinned buffers,
+    # PCIe included (reported beside the value, never as the value)
+    pcie = None
+    if args.pcie_steps > 0:
+        out_h = np.zeros(N)
+        plan.execute(u, d, out=out_h)
+        t0 = time.perf_counter()
+        for _ in range(args.pcie_steps):
+            plan.execute(u, d, out=out_h)
+        pcie = round(args.pcie_steps / (time.perf_counter() - t0), 2)
+        log(f"[bench] host-pointer execute (PCIe inclusive): {pcie} transforms/s")
+
+    # the same transform at the published figure's leaf size
+    cmp_line = None
+    if args.compare_maxs and args.compare_maxs != maxs:
+        p2 = amd.FMMPlan(N, args.compare_maxs, M, d)
+        for _ in range(args.warmup):
+            p2.execute_device(din.ptr, dout.ptr, d)
+        p2.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(args.steps):
+            p2.execute_device(din.ptr, dout.ptr, d)
+        p2.synchronize()
+        v2 = args.steps / (time.perf_counter() - t0)
+        g2 = dout.download()
+        o2 = orc.FMMPlan(N, args.compare_maxs, M, d).execute(u, d)
+        e2 = float(np.max(np.abs(g2 - o2)) / np.max(np.abs(o2)))
+        cmp_line = {"maxs": args.compare_maxs, "s": p2.info()["s"], "value": round(v2, 2),
+                    "vs_published": round(v2 / PUBLISHED["value"], 1), "max_rel_err_vs_oracle": e2}
+        log(f"[bench] maxs={args.compare_maxs}: {cmp_line}")
+        del p2
+
     cpu = None
     if not args.no_cpu:
         nt = orc.num_threads()
@@ -280,7 +321,8 @@ def main():
     line = {
         "metric": METRIC, "value": round(value, 2), "unit": "transforms/s", "n_gpus": 1,
         "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(ms_per_step, 5),
-        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f64",
+        "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": round(value / PUBLISHED["value"], 1), "baseline": PUBLISHED, "dtype": "f64",
         "data": "synthetic (SplitMix64 U(-1,1) coefficients, seed 1)",
         "config": {"workload": workload, "N": N, "maxs": maxs, "s": info["s"], "L": info["L"],
                    "M": M, "direction": "L2C" if d == 0 else "C2L", "form": "plain-fp64",
@@ -290,8 +332,16 @@ def main():
         "kernels_us": {k: round(v["avg_us"], 3) for k, v in kernels.items()},
         "roofline": roofline,
         "cpu_baseline": cpu,
+        "host_pointer_transforms_s": pcie,
+        "at_published_leaf_size": cmp_line,
     }
     print(json.dumps(line), flush=True)
+    bad = not (err_fmm <= tol and err_direct <= tol)
+    if cmp_line is not None and not cmp_line["max_rel_err_vs_oracle"] <= tol:
+        bad = True
+    if bad:
+        log(f"[bench] FAILED: error above {tol:.2e}")
+        sys.exit(1)
 
 
 if __name__ == "__main__":
