@@ -146,6 +146,52 @@ int l2c_execute_lines_device(fmm_plan *plan, const double *d_in, double *d_out,
                              size_t line_stride0, size_t line_stride1,
                              size_t elem_stride, void *stream);
 
+/* Batches of independent transforms of uneven length (no counterpart in the
+ * reference, whose callers loop over execute(); src/C/leg2cheb.c:1272-1293
+ * is the same loop for the lines of a 2-D array).
+ *
+ * l2c_create_batch plans `count` transforms of lengths n[0..count).  Their
+ * inputs and outputs are packed back to back in that order: transform t
+ * occupies elements [off_t, off_t + n[t]) with off_t = n[0] + ... + n[t-1].
+ * Transforms are grouped into size classes (n in (2^(k-1), 2^k]); each class
+ * runs as one batched launch sequence of the direct O(N^2) sum or of the FMM
+ * with leaf size s in {16, 32, 64} on the zero-padded length 2^k, so that a
+ * class shares one set of tables.  tune = 0 picks the method per class by a
+ * fixed rule (direct for 2^k <= 4 maxs with maxs = 64, else the FMM with
+ * s = 64); tune = 1 times every candidate on the class's own lines at plan
+ * time and keeps the fastest.  M is the rank (18 as in the reference).
+ * direction: L2C, C2L or BOTH.  Returns NULL on error. */
+typedef struct l2c_batch_plan_s l2c_batch_plan;
+l2c_batch_plan *l2c_create_batch(const size_t *n, size_t count, size_t M, size_t direction,
+                                 int tune);
+void l2c_free_batch(l2c_batch_plan *bp);
+
+/* Execute every transform of the batch: d_in, d_out are device pointers to the
+ * packed arrays (overwrite semantics).  Runs on the batch's own streams and
+ * returns without synchronising the host; a later call on `stream` (a
+ * hipStream_t, NULL = the batch's main stream) or l2c_batch_synchronize()
+ * sees the result.  With l2c_batch_graph(bp, 1) the launch sequence is
+ * captured into a HIP graph on the first call for a given (d_in, d_out,
+ * direction) and replayed by later calls.  Returns 0 or -1. */
+int l2c_execute_batch_device(l2c_batch_plan *bp, const double *d_in, double *d_out,
+                             size_t direction, void *stream);
+/* Same with host arrays (staged through pinned buffers; synchronous). */
+int l2c_execute_batch(l2c_batch_plan *bp, const double *in, double *out, size_t direction);
+int l2c_batch_graph(l2c_batch_plan *bp, int enable);
+int l2c_batch_synchronize(l2c_batch_plan *bp);
+
+/* One size class of a batch plan: its padded length, the number of transforms
+ * in it, the method chosen (0 direct, 1 FMM), the FMM leaf size s and depth
+ * L, and the tuner's time for the class in milliseconds (-1 if untuned). */
+typedef struct {
+  size_t n_max, count;
+  int method;
+  size_t s, L;
+  double ms;
+} l2c_batch_class;
+size_t l2c_batch_num_classes(const l2c_batch_plan *bp);
+int l2c_batch_get_class(const l2c_batch_plan *bp, size_t i, l2c_batch_class *out);
+
 /* Precision form of a plan.  0 = plain FP64 (default and, in this build,
  * the only form: any other value returns -1 with an error text). */
 int l2c_set_precision(fmm_plan *plan, int form);

@@ -38,6 +38,11 @@ class PlanInfo(C.Structure):
         ("subtree_up", C.c_int), ("subtree_down", C.c_int)]
 
 
+class BatchClass(C.Structure):
+    _fields_ = [("n_max", _sz), ("count", _sz), ("method", C.c_int), ("s", _sz), ("L", _sz),
+                ("ms", C.c_double)]
+
+
 _lib = None
 
 
@@ -107,6 +112,21 @@ def lib():
         L.l2c_memcpy_d2h.argtypes = [vp, vp, _sz]
         L.l2c_memcpy_d2d.restype = C.c_int
         L.l2c_memcpy_d2d.argtypes = [vp, vp, _sz]
+        L.l2c_create_batch.restype = vp
+        L.l2c_create_batch.argtypes = [C.POINTER(_sz), _sz, _sz, _sz, C.c_int]
+        L.l2c_free_batch.argtypes = [vp]
+        L.l2c_execute_batch_device.restype = C.c_int
+        L.l2c_execute_batch_device.argtypes = [vp, vp, vp, _sz, vp]
+        L.l2c_execute_batch.restype = C.c_int
+        L.l2c_execute_batch.argtypes = [vp, vp, vp, _sz]
+        L.l2c_batch_graph.restype = C.c_int
+        L.l2c_batch_graph.argtypes = [vp, C.c_int]
+        L.l2c_batch_synchronize.restype = C.c_int
+        L.l2c_batch_synchronize.argtypes = [vp]
+        L.l2c_batch_num_classes.restype = _sz
+        L.l2c_batch_num_classes.argtypes = [vp]
+        L.l2c_batch_get_class.restype = C.c_int
+        L.l2c_batch_get_class.argtypes = [vp, _sz, C.POINTER(BatchClass)]
         _lib = L
     return _lib
 
@@ -321,6 +341,65 @@ def LambdaI(k):
     return lib().LambdaI(int(k))
 
 
+class BatchPlan:
+    """Many independent transforms of uneven length (l2c_create_batch).
+
+    Inputs and outputs are packed back to back in the order of `lengths`.
+    tune=True times every candidate method per size class at plan time."""
+
+    def __init__(self, lengths, M=18, direction=BOTH, tune=True):
+        n = np.ascontiguousarray(lengths, dtype=np.uint64)
+        self.lengths = n.astype(np.int64)
+        self.total = int(n.sum())
+        self.offsets = np.concatenate([[0], np.cumsum(self.lengths)[:-1]]).astype(np.int64)
+        self._plan = lib().l2c_create_batch(n.ctypes.data_as(C.POINTER(_sz)), n.size, M,
+                                            direction, 1 if tune else 0)
+        if not self._plan:
+            raise L2CError("l2c_create_batch: " + last_error())
+
+    def execute(self, u, direction, out=None):
+        u = np.ascontiguousarray(u, dtype=np.float64)
+        if u.size != self.total:
+            raise ValueError(f"packed input must hold {self.total} values")
+        if out is None:
+            out = np.zeros_like(u)
+        if lib().l2c_execute_batch(self._plan, _ptr(u), _ptr(out), direction) != 0:
+            raise L2CError("l2c_execute_batch: " + last_error())
+        return out
+
+    def execute_device(self, d_in, d_out, direction, stream=None):
+        if lib().l2c_execute_batch_device(self._plan, d_in, d_out, direction, stream) != 0:
+            raise L2CError("l2c_execute_batch_device: " + last_error())
+
+    def graph(self, enable=True):
+        if lib().l2c_batch_graph(self._plan, 1 if enable else 0) != 0:
+            raise L2CError(last_error())
+
+    def synchronize(self):
+        if lib().l2c_batch_synchronize(self._plan) != 0:
+            raise L2CError(last_error())
+
+    def classes(self):
+        out = []
+        for i in range(lib().l2c_batch_num_classes(self._plan)):
+            c = BatchClass()
+            if lib().l2c_batch_get_class(self._plan, i, C.byref(c)) != 0:
+                raise L2CError(last_error())
+            out.append({"n_max": c.n_max, "count": c.count,
+                        "method": "direct" if c.method == 0 else "fmm", "s": c.s, "L": c.L,
+                        "ms": c.ms})
+        return out
+
+    def split(self, packed):
+        """The per-transform views of a packed array."""
+        return [packed[o:o + n] for o, n in zip(self.offsets, self.lengths)]
+
+    def __del__(self):
+        if getattr(self, "_plan", None):
+            lib().l2c_free_batch(self._plan)
+            self._plan = None
+
+
 class DeviceBuffer:
     """A device allocation made by the library (for device-resident runs)."""
 
@@ -370,4 +449,21 @@ def coefficients(n, seed, decay=0.0):
     u = splitmix_uniform(n, seed)
     if decay:
         u = u / np.power(np.arange(1, n + 1, dtype=np.float64), decay)
+    return u
+
+
+def log_uniform_lengths(count, seed, lo=64, hi=8193):
+    """floor(exp(U(ln lo, ln hi))) for `count` transforms (SplitMix64 `seed`)."""
+    x = splitmix_uniform(count, seed, 0.0, 1.0)
+    return np.floor(np.exp(np.log(lo) + x * (np.log(hi) - np.log(lo)))).astype(np.int64)
+
+
+def packed_coefficients(lengths, seed, decay=1.0):
+    """Packed inputs of a batch: transform t gets U(-1,1) * (k+1)^-decay, k < n_t,
+    all drawn from one SplitMix64 stream."""
+    lengths = np.asarray(lengths, dtype=np.int64)
+    u = splitmix_uniform(int(lengths.sum()), seed)
+    if decay:
+        local = np.arange(u.size) - np.repeat(np.cumsum(lengths) - lengths, lengths)
+        u = u / np.power(local + 1.0, decay)
     return u

@@ -94,6 +94,7 @@ __global__ void __launch_bounds__(kUpNT) k_up(const double *__restrict__ in, Lin
   const int j = blockIdx.x;
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   const double *src = in + line_off(lm, blockIdx.y);
+  const size_t Nl = (size_t)line_len(lm, blockIdx.y, g.N);  // zeros past the line's end
   double *ws = wsall + blockIdx.y * wsstride;
   const WsLayout lay = ws_layout(g);
   double *w = ws + lay.w;
@@ -112,7 +113,7 @@ __global__ void __launch_bounds__(kUpNT) k_up(const double *__restrict__ in, Lin
   for (int q = 0; q < 8; q++) {
     const int e = q * kUpNT + tid;
     const size_t i = base + e;
-    vin[q] = (e < nin && i < (size_t)g.N) ? src[i * lm.es] : 0.0;
+    vin[q] = (e < nin && i < Nl) ? src[i * lm.es] : 0.0;
   }
   // B operand of this wave's first MFMA task (16 k-steps = the first 64 rows of T)
   const int ntile = (M + 15) / 16;
@@ -151,7 +152,7 @@ __global__ void __launch_bounds__(kUpNT) k_up(const double *__restrict__ in, Lin
     const int e = e0 + tid;
     if (e < nin) {
       const size_t i = base + e;
-      const double v = i < (size_t)g.N ? src[i * lm.es] : 0.0;
+      const double v = i < Nl ? src[i * lm.es] : 0.0;
       xs[(e & 1) * 16 * xst + (e >> 1) / s * xst + (e >> 1) % s] =
           (dir == C2L && i > 0) ? v * (double)i : v;
     }
@@ -1007,7 +1008,6 @@ __global__ void __launch_bounds__(kDownNT) __attribute__((amdgpu_waves_per_eu(2)
 // Far field: the subtree's local expansions evaluated at the leaf nodes on
 // v_mfma_f64_16x16x4f64, F_par (s x G) = T_par (s x M) C_par (M x G), added to
 // the band sums staged in LDS; then scaling and one coalesced store.
-constexpr int kD8MaxG = 16;
 struct Down8Lds {
   int xs, nxs, lw, LW, hd, dl, cres, mres, qp, total;
 };
@@ -1131,6 +1131,8 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) k
   extern __shared__ double sm[];
   const int M = MT > 0 ? MT : g.M;
   const int s = g.s, N = g.N, M2 = 2 * M, MM = M * M;
+  // N: the plan geometry; Nl: this line's length (inputs past it read as zero)
+  const int Nl = line_len(lm, blockIdx.y, N);
   const int G = 1 << g.gd, sh = 31 - __builtin_clz(s);
   const int nw = s >> 4, NT = 64 * nw;
   const int Rr = blockIdx.x;
@@ -1140,8 +1142,8 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) k
   const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
   const bool do_band = true, do_far = mode == K8_FULL;
   const size_t row0 = (size_t)Rr * G * 2 * s;
-  if (row0 >= (size_t)N) return;  // the whole work-group
-  const int nr = (int)std::min((size_t)G * 2 * s, (size_t)N - row0);
+  if (row0 >= (size_t)Nl) return;  // the whole work-group
+  const int nr = (int)std::min((size_t)G * 2 * s, (size_t)Nl - row0);
   const size_t hc0 = (size_t)Rr * G * s;  // first half index of the work-group
   const size_t off = line_off(lm, blockIdx.y);
   const double *src = in + off;
@@ -1205,7 +1207,7 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) k
   if (do_band) {
 #pragma unroll
     for (int q = 0; q < QX; q++)
-      vx[q] = src[std::min(row0 + q * NT + tid, (size_t)N - 1) * lm.es];
+      vx[q] = src[std::min(row0 + q * NT + tid, (size_t)Nl - 1) * lm.es];
 #pragma unroll
     for (int q = 0; q < QLS; q++)
       vl[q] = tab[std::min(2 * hc0 + q * NT + tid, (size_t)N - 1)];
@@ -1261,14 +1263,14 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) k
       const int e = q * NT + tid;
       if (e < nX) {
         const size_t i = row0 + e;
-        const double u = i < (size_t)N ? vx[q] : 0.0;
+        const double u = i < (size_t)Nl ? vx[q] : 0.0;
         const int h = e >> 1;
         xs[(e & 1) * ly.nxs + h + (h >> sh)] = (DIR == 1 && i > 0) ? u * (double)i : u;  // v_i = i u_i
       }
     }
     for (int e = QX * NT + tid; e < nX; e += NT) {
       const size_t i = row0 + e;
-      double u = i < (size_t)N ? src[i * lm.es] : 0.0;
+      double u = i < (size_t)Nl ? src[i * lm.es] : 0.0;
       if (DIR == 1 && i > 0) u *= (double)i;
       const int h = e >> 1;
       xs[(e & 1) * ly.nxs + h + (h >> sh)] = u;
@@ -1408,6 +1410,7 @@ __global__ void __launch_bounds__(256) k_direct(const double *__restrict__ in,
                                                 const double *__restrict__ an,
                                                 const double *__restrict__ dn) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  N = line_len(lm, blockIdx.y, N);  // this line's length (tables cover the plan's N)
   if (i >= N) return;
   const size_t off = line_off(lm, blockIdx.y);
   const double *u = in + off;
@@ -1475,6 +1478,10 @@ static int launch_fmm_m(fmm_plan_s *p, const double *d_in, double *d_out, size_t
                         double *ws) {
   const Geo &g = p->geo;
   const size_t wsstride = ws_doubles(g);
+  if (lm.n && !g.fast) {  // per-line lengths: k_up and k_down8 only
+    set_error("lines of uneven length need a plan with leaf size s = 16, 32 or 64");
+    return -1;
+  }
   if (!band_only) {
     dim3 gu((unsigned)((size_t)4 << g.lu), (unsigned)nlines);
     prof_begin(p->prof, K_UP_SUBTREE, st);
@@ -1539,6 +1546,37 @@ int run_direct(const direct_plan_s *dp, const double *d_in, double *d_out, size_
   prof_end(dp->prof, st);
   L2C_HIP(hipGetLastError(), -1);
   return 0;
+}
+
+// Lines of uneven length packed back to back: line l (l < nlines) starts at
+// d_off[l] and has d_n[l] <= plan N elements (unit stride).  FMM plans
+// (leaf size 16, 32 or 64) or direct plans (L = 0).
+int run_packed(fmm_plan_s *p, const direct_plan_s *dp, const double *d_in, double *d_out,
+               size_t dir, size_t nlines, const size_t *d_off, const int *d_n, hipStream_t st) {
+  for (size_t l0 = 0; l0 < nlines; l0 += 65535) {
+    const size_t nl = std::min<size_t>(65535, nlines - l0);
+    LineMap lm{1, 0, 0, 1, d_off + l0, d_n + l0};
+    if (dp) {
+      const int N = (int)dp->N;
+      k_direct<<<dim3((N + 255) / 256, (unsigned)nl), 256, 0, st>>>(d_in, d_out, lm, N, (int)dir,
+                                                                   dp->d_a, dp->d_an, dp->d_dn);
+      L2C_HIP(hipGetLastError(), -1);
+    } else {
+      if (p->wl_cap < nl * ws_doubles(p->geo)) {
+        set_error("run_packed: workspace not reserved for this many lines");
+        return -1;
+      }
+      if (launch_fmm(p, d_in, d_out, dir, lm, nl, st, 0)) return -1;
+    }
+  }
+  return 0;
+}
+
+// Grow the plan's line workspace to nlines lines now (so that later launches,
+// graph capture included, never allocate).
+int reserve_lines(fmm_plan_s *p, size_t nlines) {
+  double *w = nullptr;
+  return ensure_ws(p, std::min<size_t>(nlines, 65535), &w);
 }
 
 int run_lines(fmm_plan_s *p, const double *d_in, double *d_out, size_t dir, size_t nlines,

@@ -85,11 +85,19 @@ __host__ __device__ inline size_t slot_base(int level) { return ((size_t)4 << le
 
 // Line l of a batch starts at (l / n0) * ls1 + (l % n0) * ls0; its element i
 // is at that offset + i * es.  A single transform is {1, 0, 0, stride}.
+// Batches of uneven lengths set off (device array: start of line l) and n
+// (device array: length of line l <= the plan's N); the kernels then read
+// zeros past a line's end and store only its first n[l] outputs.
 struct LineMap {
   size_t n0, ls0, ls1, es;
+  const size_t *off = nullptr;
+  const int *n = nullptr;
 };
 __host__ __device__ inline size_t line_off(const LineMap &m, size_t l) {
-  return (l / m.n0) * m.ls1 + (l % m.n0) * m.ls0;
+  return m.off ? m.off[l] : (l / m.n0) * m.ls1 + (l % m.n0) * m.ls0;
+}
+__host__ __device__ inline int line_len(const LineMap &m, size_t l, int N) {
+  return m.n ? m.n[l] : N;
 }
 
 struct Geo {
@@ -181,6 +189,9 @@ int run_direct(const direct_plan_s *dp, const double *d_in, double *d_out, size_
                size_t stride, hipStream_t st, size_t precompute);
 int run_lines(fmm_plan_s *p, const double *d_in, double *d_out, size_t dir, size_t nlines,
               size_t n0, size_t ls0, size_t ls1, size_t es, hipStream_t st);
+int run_packed(fmm_plan_s *p, const direct_plan_s *dp, const double *d_in, double *d_out,
+               size_t dir, size_t nlines, const size_t *d_off, const int *d_n, hipStream_t st);
+int reserve_lines(fmm_plan_s *p, size_t nlines);
 size_t flops_fmm(const fmm_plan_s *p, size_t dir);
 size_t down_shared_bytes(const Geo &g);  // of the downward kernel g selects
 bool down8_eligible(const Geo &g);
@@ -189,5 +200,7 @@ size_t ws_doubles(const Geo &g);  // execute workspace per line
 size_t flops_direct(size_t N, size_t dir, size_t precompute);
 // implemented in l2c_plan.hip
 direct_plan_s *make_direct(size_t N, size_t direction, size_t precompute, bool staging);
+fmm_plan_s *create_fmm_impl(size_t N, size_t maxs, size_t M, size_t direction, size_t lagrange,
+                            size_t precompute, size_t v);
 void destroy_direct(direct_plan_s *dp);
 }  // namespace l2c
