@@ -231,9 +231,96 @@ def bench_batch(amd, orc, args):
     return out
 
 
+def bench_field(amd, args):
+    """A 256^3 FP64 field c_ijk = U(-1,1)/((i+1)(j+1)(k+1)) resident in HBM:
+    L2C along axes 2, 1, 0 (65,536 strided lines of N = 256 per axis, one
+    batched call each), then C2L back along 0, 1, 2; 64 vectors of N = 2^18
+    (L2C) run at the same time on the batch plan's own streams.  One step is
+    the six field passes plus the 64 vectors.  Prints one JSON line."""
+    from oracle import orc
+    n = args.field_n
+    shape = (n, n, n)
+    rng_u = amd.splitmix_uniform(n ** 3, 3)
+    i, j, k = np.meshgrid(*[np.arange(n, dtype=np.float64)] * 3, indexing="ij")
+    x = (rng_u.reshape(shape) / ((i + 1) * (j + 1) * (k + 1))).ravel()
+    del i, j, k
+    fp = amd.FMMPlan(n, 64, args.M, amd.BOTH)
+    a, b = amd.DeviceBuffer(x.size), amd.DeviceBuffer(x.size)
+    a.upload(x)
+    # line maps (n0, ls0, ls1, es) of the three axes of a C-ordered n^3 array
+    maps = {2: (n * n, n, 0, 1), 1: (n, 1, n * n, n), 0: (n * n, 1, 0, n * n)}
+    nv, nvec = args.field_vectors, 1 << 18
+    bp = amd.BatchPlan([nvec] * nv, args.M, amd.L2C, tune=False) if nv else None
+    uv = amd.packed_coefficients([nvec] * nv, seed=4) if nv else None
+    if nv:
+        vin, vout = amd.DeviceBuffer(nv * nvec), amd.DeviceBuffer(nv * nvec)
+        vin.upload(uv)
+
+    def step():
+        if nv:
+            bp.execute_device(vin.ptr, vout.ptr, 0)
+        src, dst = a, b
+        for d, axes in ((0, (2, 1, 0)), (1, (0, 1, 2))):
+            for ax in axes:
+                n0, ls0, ls1, es = maps[ax]
+                fp.execute_lines_device(src.ptr, dst.ptr, d, n * n, n0, ls0, ls1, es)
+                src, dst = dst, src
+        fp.synchronize()
+        if nv:
+            bp.synchronize()
+
+    step()
+    # correctness: the first L2C pass (axis 2) on sampled lines, the round trip,
+    # two of the 2^18 vectors
+    fp.execute_lines_device(a.ptr, b.ptr, 0, n * n, *maps[2])
+    fp.synchronize()
+    g = b.download().reshape(n * n, n)
+    op = orc.FMMPlan(n, 64, args.M, orc.BOTH)
+    xl = x.reshape(n * n, n)
+    rows = np.random.default_rng(1).choice(n * n, 64, replace=False)
+    err_lines = max(float(np.max(np.abs(g[r] - op.execute(xl[r], 0))) / np.max(np.abs(op.execute(xl[r], 0))))
+                    for r in rows)
+    step()  # a -> ... -> a after six passes
+    rt = float(np.max(np.abs(a.download() - x)))
+    a.upload(x)
+    err_vec = 0.0
+    if nv:
+        gv = vout.download()
+        ov = orc.FMMPlan(nvec, 64, args.M, 0)
+        for t in (0, nv - 1):
+            ref = ov.execute(uv[t * nvec:(t + 1) * nvec], 0)
+            err_vec = max(err_vec, float(np.max(np.abs(gv[t * nvec:(t + 1) * nvec] - ref)) / np.max(np.abs(ref))))
+    for _ in range(args.warmup // 10 + 2):
+        step()
+    steps = max(5, args.steps // 20)
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        step()
+    el = (time.perf_counter() - t0) / steps
+    lines = 6 * n * n
+    out = {"metric": "field lines/s: 256^3 FP64 field, L2C along 3 axes then C2L back, "
+                     "with 64 x 2^18 L2C transforms mixed in",
+           "value": round(lines / el, 1), "unit": "lines/s", "n_gpus": 1, "steps": steps,
+           "ms_per_step": round(1e3 * el, 4), "higher_is_better": True, "dtype": "f64",
+           "data": "synthetic (SplitMix64)", "transforms_per_s": round((lines + nv) / el, 1),
+           "config": {"workload": f"field{n}_x6_plus_{nv}x2^18", "field": list(shape),
+                      "vectors": nv, "vector_n": nvec, "M": args.M},
+           "max_rel_err": {"field_lines_vs_oracle": err_lines, "vectors_vs_oracle": err_vec},
+           "round_trip_max_abs_err": rt}
+    print(json.dumps(out), flush=True)
+    if not (err_lines <= 1e-14 * math.log2(n) and err_vec <= 1e-14 * 18 and rt < 1e-10):
+        log("[bench] FAILED: field error above tolerance")
+        sys.exit(1)
+
+
 def main():
     ap = argparse.ArgumentParser(description=__doc__, formatter_class=argparse.RawDescriptionHelpFormatter)
     ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--workload", choices=("fmm", "field"), default="fmm",
+                    help="fmm: the N=2^20 transform and the uneven batch (default); field: the "
+                         "256^3 strided field with 2^18 vectors mixed in")
+    ap.add_argument("--field-n", type=int, default=256)
+    ap.add_argument("--field-vectors", type=int, default=64)
     ap.add_argument("--steps", type=int, default=400)
     ap.add_argument("--warmup", type=int, default=40)
     ap.add_argument("--N", type=int, default=1 << 20)
@@ -278,10 +365,13 @@ def main():
     world = int(os.environ.get("WORLD_SIZE", "1"))
     if args.gpus != 1 or world != 1:
         print("bench.py: this build runs on one GPU only; --gpus must be 1 "
-              "(sharding across GPUs is described in DESIGN.md, not launched).", file=sys.stderr)
+              "(no multi-GPU path is launched).", file=sys.stderr)
         sys.exit(2)
 
     amd = load_pkg()
+    if args.workload == "field":
+        bench_field(amd, args)
+        return
     N, maxs, M, d = args.N, args.maxs, args.M, args.direction
     u = amd.coefficients(N, seed=1, decay=args.decay)
 

@@ -335,17 +335,18 @@ __device__ __forceinline__ void m2l_block_compute(const double *Al, const double
 template <int MT>
 __global__ void __launch_bounds__(kM2lNT) k_m2l(Geo g, const double *__restrict__ A,
                                                 double *__restrict__ wsall, size_t wsstride,
-                                                int bpw) {
+                                                int bpw, size_t nlines, int lpb) {
   extern __shared__ double sm[];
   const int M = MT > 0 ? MT : g.M, M2 = 2 * M, MM = M * M, Mp = M + 1;
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-  double *ws = wsall + blockIdx.y * wsstride;
   const WsLayout lay = ws_layout(g);
-  const double *w = ws + lay.w;
-  double *mo = ws + lay.m2l;
   const int ntop = (int)total_blocks(g.lu);
   const int ntot = (int)total_blocks(g.L);
   const int WL = m2l_wave_lds(M);
+  // lines line0 .. line0 + nl - 1 share every staged block (the plan's M2L
+  // matrices are read once per work-group, not once per line)
+  const size_t line0 = (size_t)blockIdx.y * lpb;
+  const int nl = (int)std::min<size_t>((size_t)lpb, nlines - line0);
 
   if ((int)blockIdx.x < ntop || bpw == 1) {
     // ---- one block per work-group: every block above lu (multipoles summed
@@ -363,51 +364,57 @@ __global__ void __launch_bounds__(kM2lNT) k_m2l(Geo g, const double *__restrict_
       const int e = q * kM2lNT + tid;
       va[q] = e < 3 * MM ? Ab[e] : 0.0;
     }
-    const int nw = 2 * M2;  // multipole doubles of chunks 2b+2, 2b+3
-    if (lev >= g.lu) {
-      const double *wb = w + (slot_base(lev) + 2 * (size_t)b + 2) * M2;
-      for (int o = tid; o < nw; o += kM2lNT) wl[o] = wb[o];
-    } else {
-      const size_t nd = (size_t)1 << (g.lu - lev);
-      const int ngrp = std::max(1, kM2lNT / nw);
-      for (int o0 = 0; o0 < nw; o0 += kM2lNT) {
-        const int o = o0 + tid % std::min(nw, kM2lNT), grp = tid / std::min(nw, kM2lNT);
-        double acc = 0;
-        if (grp < ngrp && o < nw) {
-          const int ci = o / M2, oo = o % M2;
-          const double *pp = ws + lay.part +
-                             ((size_t)lev * ((size_t)4 << g.lu) + (2 * (size_t)b + 2 + ci) * nd) * M2 + oo;
-          for (size_t d0 = grp; d0 < nd; d0 += 8 * (size_t)ngrp) {
-            double v[8];
-#pragma unroll
-            for (int q = 0; q < 8; q++) {
-              const size_t d = d0 + (size_t)q * ngrp;
-              v[q] = d < nd ? pp[d * M2] : 0.0;
-            }
-#pragma unroll
-            for (int q = 0; q < 8; q++) acc += v[q];
-          }
-        }
-        red[tid] = acc;
-        __syncthreads();
-        if (tid < std::min(nw, kM2lNT) && o0 + tid < nw) {
-          double sum = 0;
-          for (int gi = 0; gi < ngrp; gi++) sum += red[gi * std::min(nw, kM2lNT) + tid];
-          wl[o0 + tid] = sum;
-        }
-        __syncthreads();
-      }
-    }
 #pragma unroll
     for (int q = 0; q < 8; q++) {
       const int e = q * kM2lNT + tid;
       if (e < 3 * MM) Al[(e / M) * Mp + e % M] = va[q];
     }
     for (int e = 8 * kM2lNT + tid; e < 3 * MM; e += kM2lNT) Al[(e / M) * Mp + e % M] = Ab[e];
-    __syncthreads();
-    m2l_block_compute<MT, true>(Al, wl, wl + nw, M, tid,
-                                mo + (slot_base(lev) + 2 * (size_t)b) * M2,
-                                mo + (slot_base(lev) + 2 * (size_t)b + 1) * M2);
+    const int nw = 2 * M2;  // multipole doubles of chunks 2b+2, 2b+3
+    for (int li = 0; li < nl; li++) {
+      double *ws = wsall + (line0 + li) * wsstride;
+      const double *w = ws + lay.w;
+      double *mo = ws + lay.m2l;
+      if (lev >= g.lu) {
+        const double *wb = w + (slot_base(lev) + 2 * (size_t)b + 2) * M2;
+        for (int o = tid; o < nw; o += kM2lNT) wl[o] = wb[o];
+      } else {
+        const size_t nd = (size_t)1 << (g.lu - lev);
+        const int ngrp = std::max(1, kM2lNT / nw);
+        for (int o0 = 0; o0 < nw; o0 += kM2lNT) {
+          const int o = o0 + tid % std::min(nw, kM2lNT), grp = tid / std::min(nw, kM2lNT);
+          double acc = 0;
+          if (grp < ngrp && o < nw) {
+            const int ci = o / M2, oo = o % M2;
+            const double *pp = ws + lay.part +
+                               ((size_t)lev * ((size_t)4 << g.lu) + (2 * (size_t)b + 2 + ci) * nd) * M2 + oo;
+            for (size_t d0 = grp; d0 < nd; d0 += 8 * (size_t)ngrp) {
+              double v[8];
+#pragma unroll
+              for (int q = 0; q < 8; q++) {
+                const size_t d = d0 + (size_t)q * ngrp;
+                v[q] = d < nd ? pp[d * M2] : 0.0;
+              }
+#pragma unroll
+              for (int q = 0; q < 8; q++) acc += v[q];
+            }
+          }
+          red[tid] = acc;
+          __syncthreads();
+          if (tid < std::min(nw, kM2lNT) && o0 + tid < nw) {
+            double sum = 0;
+            for (int gi = 0; gi < ngrp; gi++) sum += red[gi * std::min(nw, kM2lNT) + tid];
+            wl[o0 + tid] = sum;
+          }
+          __syncthreads();
+        }
+      }
+      __syncthreads();
+      m2l_block_compute<MT, true>(Al, wl, wl + nw, M, tid,
+                                  mo + (slot_base(lev) + 2 * (size_t)b) * M2,
+                                  mo + (slot_base(lev) + 2 * (size_t)b + 1) * M2);
+      __syncthreads();  // wl and the result scratch are reused by the next line
+    }
     return;
   }
 
@@ -419,7 +426,8 @@ __global__ void __launch_bounds__(kM2lNT) k_m2l(Geo g, const double *__restrict_
   const int b = B - (int)total_blocks(lev);
   double *Al = sm + wave * WL, *wl = Al + 3 * M * Mp;
   const double *Ab = A + (size_t)(3 * total_blocks(lev) + 3 * b) * MM;
-  const double *wb = w + (slot_base(lev) + 2 * (size_t)b + 2) * M2;  // chunks 2b+2, 2b+3
+  const size_t wofs = lay.w + (slot_base(lev) + 2 * (size_t)b + 2) * M2;  // chunks 2b+2, 2b+3
+  const size_t oofs = lay.m2l + (slot_base(lev) + 2 * (size_t)b) * M2;
   constexpr int QA = MT > 0 ? (3 * MT * MT + 63) / 64 : 16;
   double va[QA];
 #pragma unroll
@@ -427,22 +435,32 @@ __global__ void __launch_bounds__(kM2lNT) k_m2l(Geo g, const double *__restrict_
     const int e = q * 64 + lane;
     va[q] = e < 3 * MM ? Ab[e] : 0.0;
   }
-  const double w0v = lane < 2 * M2 ? wb[lane] : 0.0;  // 2*M2 <= 192
-  const double w1v = lane + 64 < 2 * M2 ? wb[lane + 64] : 0.0;
-  const double w2v = lane + 128 < 2 * M2 ? wb[lane + 128] : 0.0;
+  // multipoles of the first line; later lines are fetched one line ahead
+  const double *wb = wsall + line0 * wsstride + wofs;
+  double w0v = lane < 2 * M2 ? wb[lane] : 0.0;  // 2*M2 <= 192
+  double w1v = lane + 64 < 2 * M2 ? wb[lane + 64] : 0.0;
+  double w2v = lane + 128 < 2 * M2 ? wb[lane + 128] : 0.0;
 #pragma unroll
   for (int q = 0; q < QA; q++) {
     const int e = q * 64 + lane;
     if (e < 3 * MM) Al[(e / M) * Mp + e % M] = va[q];
   }
   for (int e = QA * 64 + lane; e < 3 * MM; e += 64) Al[(e / M) * Mp + e % M] = Ab[e];
-  if (lane < 2 * M2) wl[lane] = w0v;
-  if (lane + 64 < 2 * M2) wl[lane + 64] = w1v;
-  if (lane + 128 < 2 * M2) wl[lane + 128] = w2v;
-  wave_sync();
-  m2l_block_compute<MT, false>(Al, wl, wl + 2 * M2, M, lane,
-                               mo + (slot_base(lev) + 2 * (size_t)b) * M2,
-                               mo + (slot_base(lev) + 2 * (size_t)b + 1) * M2);
+  for (int li = 0; li < nl; li++) {
+    if (lane < 2 * M2) wl[lane] = w0v;
+    if (lane + 64 < 2 * M2) wl[lane + 64] = w1v;
+    if (lane + 128 < 2 * M2) wl[lane + 128] = w2v;
+    if (li + 1 < nl) {
+      const double *wn = wsall + (line0 + li + 1) * wsstride + wofs;
+      w0v = lane < 2 * M2 ? wn[lane] : 0.0;
+      w1v = lane + 64 < 2 * M2 ? wn[lane + 64] : 0.0;
+      w2v = lane + 128 < 2 * M2 ? wn[lane + 128] : 0.0;
+    }
+    wave_sync();
+    double *mo = wsall + (line0 + li) * wsstride + oofs;
+    m2l_block_compute<MT, false>(Al, wl, wl + 2 * M2, M, lane, mo, mo + M2);
+    wave_sync();  // wl and the result scratch are reused by the next line
+  }
 }
 
 // Near-field band for four consecutive half-rows r0..r0+3 of one parity:
@@ -1430,6 +1448,130 @@ __global__ void __launch_bounds__(256) k_direct(const double *__restrict__ in,
 }
 
 // ---------------------------------------------------------------------------
+// Batched lines of the direct path (N <= kDenseMaxN) as dense products on FP64
+// MFMA: per parity p, OUT_p (NPP x LT) = K_p (NPP x NPP) IN_p (NPP x LT) for a
+// tile of LT lines, K_p upper triangular (the direct sums of
+// src/C/leg2cheb.c:152-234 with scaling and C2L weights folded in).
+//
+// LDS holds the tile's input as [p][c][line] (row stride kDenseLTP); MFMA
+// v_mfma_f64_16x16x4f64 with A = K_p from global memory (the plan's table,
+// L2-resident), B = the input rows from LDS.  Row tiles R and NR-1-R are
+// paired so that every wave runs the same number of k-steps; a wave keeps its
+// results in registers until all waves are done with the input, then the
+// tile is written back through LDS with the loads' coalescing.
+constexpr int kDenseLT = 32, kDenseLTP = 36, kDenseNT = 256;
+__global__ void __launch_bounds__(kDenseNT) k_lines_dense(const double *__restrict__ in,
+                                                          double *__restrict__ out, LineMap lm,
+                                                          size_t nlines, int N, int NPP,
+                                                          const double *__restrict__ K) {
+  extern __shared__ double sm[];
+  const int tid = threadIdx.x, lane = tid & 63;
+  const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
+  const size_t l0 = (size_t)blockIdx.x * kDenseLT;
+  const int LT = kDenseLT, LTP = kDenseLTP, NE = 2 * NPP;
+  const bool contig = lm.es == 1;
+  // element e of the tile: (j, l) with j fastest when lines are contiguous,
+  // l fastest otherwise, so that neighbouring threads touch neighbouring words
+  const int tot = NE * LT;
+#pragma unroll 8
+  for (int e = tid; e < tot; e += kDenseNT) {
+    const int l = contig ? e / NE : e % LT;
+    const int j = contig ? e % NE : e / LT;
+    const size_t line = l0 + l;
+    double v = 0.0;
+    if (line < nlines && j < line_len(lm, line, N)) v = in[line_off(lm, line) + (size_t)j * lm.es];
+    sm[(j & 1) * NPP * LTP + (j >> 1) * LTP + l] = v;
+  }
+  __syncthreads();
+
+  const int NR = NPP >> 4, npair = (NR + 1) >> 1;
+  d4 res[2][2][2];  // [pair slot][row tile of the pair][column tile]
+  int rtile[2][2], rpar[2];
+#pragma unroll
+  for (int q = 0; q < 2; q++) {
+    rpar[q] = -1;
+    rtile[q][0] = rtile[q][1] = -1;
+#pragma unroll
+    for (int h = 0; h < 2; h++)
+#pragma unroll
+      for (int c = 0; c < 2; c++) res[q][h][c] = d4{0.0, 0.0, 0.0, 0.0};
+  }
+#pragma unroll
+  for (int q = 0; q < 2; q++) {
+    const int jp = wave + 4 * q;  // pair index: parity jp & 1, row tiles jp/2 and NR-1-jp/2
+    if (jp >= 2 * npair) break;
+    const int p = jp & 1, Rp = jp >> 1;
+    rpar[q] = p;
+    const double *Bp = sm + p * NPP * LTP + (lane >> 4) * LTP + (lane & 15);
+#pragma unroll
+    for (int h = 0; h < 2; h++) {
+      const int R = h == 0 ? Rp : NR - 1 - Rp;
+      if (h == 1 && R == Rp) break;
+      rtile[q][h] = R;
+      const int nk = (NPP - 16 * R) >> 2;  // k-steps from the diagonal tile on
+      const double *Kr = K + ((size_t)p * NPP + 16 * R + (lane & 15)) * NPP + 16 * R + (lane >> 4);
+      double av[32];
+#pragma unroll
+      for (int k = 0; k < 32; k++) av[k] = k < nk ? Kr[4 * k] : 0.0;
+      d4 D0 = {0.0, 0.0, 0.0, 0.0}, D1 = {0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+      for (int k = 0; k < 32; k++) {
+        if (k < nk) {
+          const double *b = Bp + (16 * R + 4 * k) * LTP;
+          D0 = __builtin_amdgcn_mfma_f64_16x16x4f64(av[k], b[0], D0, 0, 0, 0);
+          D1 = __builtin_amdgcn_mfma_f64_16x16x4f64(av[k], b[16], D1, 0, 0, 0);
+        }
+      }
+      res[q][h][0] = D0;
+      res[q][h][1] = D1;
+    }
+  }
+  __syncthreads();  // every wave is done with the input tile
+#pragma unroll
+  for (int q = 0; q < 2; q++) {
+    if (rpar[q] < 0) break;
+#pragma unroll
+    for (int h = 0; h < 2; h++) {
+      const int R = rtile[q][h];
+      if (R < 0) break;
+      double *o = sm + rpar[q] * NPP * LTP + (16 * R + (lane >> 4)) * LTP + (lane & 15);
+#pragma unroll
+      for (int c = 0; c < 2; c++)
+#pragma unroll
+        for (int v = 0; v < 4; v++) o[4 * v * LTP + 16 * c] = res[q][h][c][v];
+    }
+  }
+  __syncthreads();
+#pragma unroll 8
+  for (int e = tid; e < tot; e += kDenseNT) {
+    const int l = contig ? e / NE : e % LT;
+    const int j = contig ? e % NE : e / LT;
+    const size_t line = l0 + l;
+    if (line < nlines && j < line_len(lm, line, N))
+      out[line_off(lm, line) + (size_t)j * lm.es] = sm[(j & 1) * NPP * LTP + (j >> 1) * LTP + l];
+  }
+}
+
+static size_t dense_shared_bytes(int NPP) { return 8 * (size_t)2 * NPP * kDenseLTP; }
+
+// Batched lines through a direct plan: dense MFMA products when the plan has
+// them and the batch is large enough, else one row per thread.
+static int launch_direct_lines(const direct_plan_s *dp, const double *d_in, double *d_out,
+                               size_t dir, const LineMap &lm, size_t nlines, hipStream_t st) {
+  const int N = (int)dp->N;
+  if (dp->d_K[dir] && nlines >= 4) {
+    const size_t nb = (nlines + kDenseLT - 1) / kDenseLT;
+    k_lines_dense<<<dim3((unsigned)nb), kDenseNT, dense_shared_bytes(dp->NPP), st>>>(
+        d_in, d_out, lm, nlines, N, dp->NPP, dp->d_K[dir]);
+  } else {
+    k_direct<<<dim3((N + 255) / 256, (unsigned)nlines), 256, 0, st>>>(d_in, d_out, lm, N, (int)dir,
+                                                                      dp->d_a, dp->d_an, dp->d_dn);
+  }
+  L2C_HIP(hipGetLastError(), -1);
+  return 0;
+}
+
+// ---------------------------------------------------------------------------
 // host orchestration
 
 // Workspace for a batch of lines: grown on demand, owned by the plan.
@@ -1490,9 +1632,15 @@ static int launch_fmm_m(fmm_plan_s *p, const double *d_in, double *d_out, size_t
     prof_end(p->prof, st);
     const int ntop = (int)total_blocks(g.lu), ntot = (int)total_blocks(g.L);
     const int bpw = m2l_bpw(g);
-    dim3 gm((unsigned)(ntop + (ntot - ntop + bpw - 1) / bpw), (unsigned)nlines);
+    const unsigned gx = (unsigned)(ntop + (ntot - ntop + bpw - 1) / bpw);
+    // lines per work-group: share each staged block over up to 8 lines while
+    // the grid keeps at least ~2048 work-groups
+    int lpb = 1;
+    while (lpb < 8 && (size_t)gx * ((nlines + 2 * lpb - 1) / (2 * lpb)) >= 2048) lpb *= 2;
+    dim3 gm(gx, (unsigned)((nlines + lpb - 1) / lpb));
     prof_begin(p->prof, K_M2L, st);
-    k_m2l<MT><<<gm, kM2lNT, m2l_shared_bytes(g), st>>>(g, p->t.A[dir], ws, wsstride, bpw);
+    k_m2l<MT><<<gm, kM2lNT, m2l_shared_bytes(g), st>>>(g, p->t.A[dir], ws, wsstride, bpw,
+                                                      nlines, lpb);
     prof_end(p->prof, st);
   }
   if (g.fast) {
@@ -1557,10 +1705,7 @@ int run_packed(fmm_plan_s *p, const direct_plan_s *dp, const double *d_in, doubl
     const size_t nl = std::min<size_t>(65535, nlines - l0);
     LineMap lm{1, 0, 0, 1, d_off + l0, d_n + l0};
     if (dp) {
-      const int N = (int)dp->N;
-      k_direct<<<dim3((N + 255) / 256, (unsigned)nl), 256, 0, st>>>(d_in, d_out, lm, N, (int)dir,
-                                                                   dp->d_a, dp->d_an, dp->d_dn);
-      L2C_HIP(hipGetLastError(), -1);
+      if (launch_direct_lines(dp, d_in, d_out, dir, lm, nl, st)) return -1;
     } else {
       if (p->wl_cap < nl * ws_doubles(p->geo)) {
         set_error("run_packed: workspace not reserved for this many lines");
@@ -1586,6 +1731,12 @@ int run_lines(fmm_plan_s *p, const double *d_in, double *d_out, size_t dir, size
   if (p->L == 0) {
     const direct_plan_s *dp = p->dplan;
     const int N = (int)dp->N;
+    if (dp->d_K[dir] && nlines >= 4) {
+      prof_begin(dp->prof, K_DIRECT, st);
+      const int rc = launch_direct_lines(dp, d_in, d_out, dir, lm, nlines, st);
+      prof_end(dp->prof, st);
+      return rc;
+    }
     for (size_t l0 = 0; l0 < nlines; l0 += 65535) {
       const size_t nl = std::min<size_t>(65535, nlines - l0);
       if (l0 % lm.n0 != 0 && nlines > 65535) {

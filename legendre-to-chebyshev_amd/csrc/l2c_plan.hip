@@ -142,6 +142,34 @@ __global__ void k_fill_dn(const double *a, double *dn, int n) {
   dn[j] = j == 0 ? 0.0 : a[j - 1] / (double)(2 * j);
 }
 
+// Dense parity blocks of the direct sums (src/C/leg2cheb.c:152-234) for
+// batched lines: out[2r+p] = sum_c K[p][r][c] u[2c+p], r, c < NPP, with the
+// final scaling (2/pi and b_0/2 for L2C; (i+1/2) for C2L) and the C2L input
+// weights v_j = j u_j (v_0 = u_0) folded in; zero below the diagonal and past
+// the length.  Grid (NPP, 2), one row per work-group.
+__global__ void k_build_dense(double *K, const double *__restrict__ a,
+                              const double *__restrict__ an, const double *__restrict__ dn,
+                              int N, int NPP, int dir) {
+#pragma clang fp contract(off)
+  const int r = blockIdx.x, p = blockIdx.y, i = 2 * r + p;
+  for (int c = threadIdx.x; c < NPP; c += blockDim.x) {
+    const int j = 2 * c + p;
+    double v = 0.0;
+    if (c >= r && j < N) {
+      if (dir == L2C) {
+        v = 0.63661977236758134307553505349 * a[c - r] * a[r + c + p];
+        if (i == 0) v *= 0.5;
+      } else {
+        const double wj = j == 0 ? 1.0 : (double)j;
+        v = c == r ? 1.77245385090551602729816748334 * an[i] * wj
+                   : -(dn[c - r] * an[r + c + p]) * wj;
+        v *= (double)i + 0.5;
+      }
+    }
+    K[((size_t)p * NPP + r) * NPP + c] = v;
+  }
+}
+
 // Lambda((y-x)/2) for block 0 of each level, q-p in {0,1}: it depends on y-x
 // only, so the reference computes it once per level and reuses it for every
 // block (src/C/leg2cheb.c:948-955).  Entries with j >= M-i take the value at
@@ -298,6 +326,21 @@ direct_plan_s *make_direct(size_t N, size_t direction, size_t precompute, bool s
     destroy_direct(dp);
     return nullptr;
   }
+  if (N <= (size_t)kDenseMaxN) {
+    dp->NPP = (int)(((N + 1) / 2 + 15) / 16 * 16);
+    const size_t nk = 2 * (size_t)dp->NPP * dp->NPP;
+    for (int d = 0; d < 2; d++) {
+      if ((d == L2C && direction == C2L) || (d == C2L && !c2l)) continue;
+      if (dmalloc(&dp->d_K[d], nk, nullptr)) {
+        destroy_direct(dp);
+        return nullptr;
+      }
+      k_build_dense<<<dim3(dp->NPP, 2), 64, 0, dp->stream>>>(dp->d_K[d], dp->d_a, dp->d_an,
+                                                             dp->d_dn, (int)N, dp->NPP, d);
+      L2C_HIP(hipGetLastError(), (destroy_direct(dp), nullptr));
+    }
+    L2C_HIP(hipStreamSynchronize(dp->stream), (destroy_direct(dp), nullptr));
+  }
   if (staging) {
     if (dmalloc(&dp->d_in, N, nullptr) || dmalloc(&dp->d_out, N, nullptr) ||
         hipHostMalloc((void **)&dp->h_in, N * sizeof(double)) != hipSuccess ||
@@ -315,6 +358,8 @@ void destroy_direct(direct_plan_s *dp) {
   dfree(dp->d_a);
   dfree(dp->d_an);
   dfree(dp->d_dn);
+  dfree(dp->d_K[0]);
+  dfree(dp->d_K[1]);
   dfree(dp->d_in);
   dfree(dp->d_out);
   if (dp->h_in) (void)hipHostFree(dp->h_in);

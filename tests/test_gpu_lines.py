@@ -1,0 +1,98 @@
+"""GPU parity of batched strided lines (l2c_execute_lines_device), the path of
+a spectral field transformed along each axis (the reference's execute with a
+stride, src/C/leg2cheb.c:1299-1500, and its 2-D driver :1267-1297, applied to
+every line).  Lines of the direct path (N <= 4 maxs) run as dense MFMA
+products; every line is checked against the oracle's single transform.
+Tolerance: relative max-norm error <= 1e-14 * log2(N) per line."""
+import math
+
+import numpy as np
+import pytest
+
+from conftest import announce, relerr
+
+pytestmark = pytest.mark.gpu
+
+
+def tol(n):
+    return 1e-14 * math.log2(max(n, 2))
+
+
+def along_axis(amd, x, axis, direction, maxs=64):
+    """Transform every line of the C-ordered array x along `axis` in one
+    batched call on the device."""
+    shape = x.shape
+    N = shape[axis]
+    es = int(np.prod(shape[axis + 1:]))          # element stride along the axis
+    inner = es                                   # lines inside one outer slab
+    outer = int(np.prod(shape[:axis]))
+    nlines = outer * inner
+    plan = amd.FMMPlan(N, maxs, 18, amd.BOTH)
+    din, dout = amd.DeviceBuffer(x.size), amd.DeviceBuffer(x.size)
+    din.upload(x.ravel())
+    # line l = (o, i): offset o * N * es + i
+    plan.execute_lines_device(din.ptr, dout.ptr, direction, nlines, inner, 1, N * es, es)
+    plan.synchronize()
+    return dout.download().reshape(shape)
+
+
+def oracle_along_axis(orc, x, axis, direction, maxs=64):
+    N = x.shape[axis]
+    op = orc.FMMPlan(N, maxs, 18, orc.BOTH)
+    xm = np.moveaxis(x, axis, -1)
+    out = np.empty_like(xm)
+    for idx in np.ndindex(xm.shape[:-1]):
+        out[idx] = op.execute(np.ascontiguousarray(xm[idx]), direction)
+    return np.moveaxis(out, -1, axis)
+
+
+def per_line_error(g, o, axis):
+    gm, om = np.moveaxis(g, axis, -1), np.moveaxis(o, axis, -1)
+    return max(relerr(gm[idx], om[idx]) for idx in np.ndindex(gm.shape[:-1]))
+
+
+@pytest.mark.parametrize("shape", [(40, 48, 64), (256, 7, 9), (5, 256, 33), (3, 2, 256)])
+@pytest.mark.parametrize("direction", [0, 1])
+def test_field_axes(amd, orc, shape, direction):
+    announce(f"field {shape} dir={direction}")
+    rng = np.random.default_rng(sum(shape) + direction)
+    i, j, k = np.meshgrid(*[np.arange(n) for n in shape], indexing="ij")
+    x = rng.uniform(-1, 1, shape) / ((i + 1.0) * (j + 1.0) * (k + 1.0))
+    for axis in range(3):
+        g = along_axis(amd, x, axis, direction)
+        o = oracle_along_axis(orc, x, axis, direction)
+        e = per_line_error(g, o, axis)
+        print(f"  axis {axis}: worst per-line error {e:.3e}")
+        assert e <= tol(shape[axis])
+
+
+@pytest.mark.parametrize("N", [1, 2, 3, 17, 33, 200, 255, 256])
+def test_dense_lengths(amd, orc, N):
+    """Every padding case of the dense parity blocks (row tiles 1..8, odd and
+    even tile counts) on contiguous and on strided lines."""
+    announce(f"dense lines N={N}")
+    rng = np.random.default_rng(N)
+    x = rng.uniform(-1, 1, (70, N))
+    for d in (0, 1):
+        g1 = along_axis(amd, x, 1, d)
+        o1 = oracle_along_axis(orc, x, 1, d)
+        assert per_line_error(g1, o1, 1) <= tol(N)
+        xt = np.ascontiguousarray(x.T)
+        g0 = along_axis(amd, xt, 0, d)
+        assert per_line_error(g0, np.ascontiguousarray(o1.T), 0) <= tol(N)
+
+
+def test_field_round_trip(amd):
+    """L2C along the three axes, then C2L back: the input returns
+    (src/bin/l2c.c:342, 387 thresholds)."""
+    announce("field round trip 64^3")
+    rng = np.random.default_rng(9)
+    n = 64
+    i, j, k = np.meshgrid(*[np.arange(n)] * 3, indexing="ij")
+    x = rng.uniform(-1, 1, (n, n, n)) / ((i + 1.0) * (j + 1.0) * (k + 1.0))
+    y = x
+    for axis in (2, 1, 0):
+        y = along_axis(amd, y, axis, 0)
+    for axis in (0, 1, 2):
+        y = along_axis(amd, y, axis, 1)
+    assert float(np.max(np.abs(y - x))) < 1e-10
