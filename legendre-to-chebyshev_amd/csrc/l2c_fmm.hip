@@ -76,9 +76,13 @@ __device__ __forceinline__ void lds_sync() {
 constexpr int kUpNT = 256;
 
 __host__ __device__ inline int up_xstride(int s) { return s | 1; }  // odd: conflict-free rows
+// LDS: input [2][16][xst], moments [16][2][M], parents [8+1][2][M], the
+// upward operators U(0), U(1) and the work-group's lu composite operators P,
+// rows padded to M+1
 size_t up_shared_bytes(const Geo &g) {
-  const size_t M2 = 2 * g.M;
-  return 8 * (2 * 16 * (size_t)up_xstride(g.s) + 16 * M2 + 9 * M2);
+  const size_t M2 = 2 * g.M, Mp = g.M + 1;
+  return 8 * (2 * 16 * (size_t)up_xstride(g.s) + 16 * M2 + 9 * M2 +
+              (2 + (size_t)g.lu) * g.M * Mp);
 }
 
 template <int MT>
@@ -100,9 +104,12 @@ __global__ void __launch_bounds__(kUpNT) k_up(const double *__restrict__ in, Lin
   double *w = ws + lay.w;
   double *part = ws + lay.part;
   const int xst = up_xstride(s);
+  const int Mp = M + 1;
   double *xs = sm;                  // [2][16][xst]
   double *w0 = xs + 2 * 16 * xst;   // [16][2][M]
   double *w1 = w0 + 16 * M2;        // [8+1][2][M]
+  double *Ul = w1 + 9 * M2;         // [2][M][M+1]: U(0), U(1)
+  double *Pl = Ul + 2 * M * Mp;     // [lu][M][M+1]: P(lu - l, j mod 2^(lu-l)), l < lu
 
   // ---- loads, all issued before any use: input, this thread's rows of the
   // upward operators (k = tid % M) and of its composite operator P
@@ -130,24 +137,31 @@ __global__ void __launch_bounds__(kUpNT) k_up(const double *__restrict__ in, Lin
     }
   }
   const int ku = tid % M;
-  double u0r[MT > 0 ? MT : 1], u1r[MT > 0 ? MT : 1];
   const int np = g.lu * M2;
-  double pr[MT > 0 ? MT : 1];
-  if (MT > 0) {
+  // the operators, once per work-group: U(0), U(1) and the lu composite
+  // operators, element e of the concatenation [U | P(lu) .. P(1)]; every load
+  // of the first batch is in flight before the first LDS store
+  const int nop = (2 + g.lu) * MM;
+  auto op_src = [&](int e) -> const double * {
+    if (e < 2 * MM) return U + e;
+    const int lev = e / MM - 2, r = e % MM;
+    const int k = g.lu - lev;
+    const size_t d = (size_t)j & (((size_t)1 << k) - 1);
+    return P + ((((size_t)1 << k) - 2) + d) * MM + r;
+  };
+  constexpr int QO = 12;
+  double vo[QO];
 #pragma unroll
-    for (int l = 0; l < (MT > 0 ? MT : 1); l++) {
-      u0r[l] = U[ku * M + l];
-      u1r[l] = U[MM + ku * M + l];
-    }
-    if (tid < np) {
-      const int lev = tid / M2, i = tid % M;
-      const int k = g.lu - lev;
-      const size_t d = (size_t)j & (((size_t)1 << k) - 1);
-      const double *prow = P + ((((size_t)1 << k) - 2) + d) * MM + (size_t)i * M;
-#pragma unroll
-      for (int l = 0; l < (MT > 0 ? MT : 1); l++) pr[l] = prow[l];
-    }
+  for (int q = 0; q < QO; q++) {
+    const int e = q * kUpNT + tid;
+    vo[q] = e < nop ? *op_src(e) : 0.0;
   }
+#pragma unroll
+  for (int q = 0; q < QO; q++) {
+    const int e = q * kUpNT + tid;
+    if (e < nop) Ul[(e / M) * Mp + e % M] = vo[q];  // Pl follows Ul with the same row stride
+  }
+  for (int e = QO * kUpNT + tid; e < nop; e += kUpNT) Ul[(e / M) * Mp + e % M] = *op_src(e);
   for (int e0 = 8 * kUpNT; e0 < nin; e0 += kUpNT) {  // rare: G*2s > 2048
     const int e = e0 + tid;
     if (e < nin) {
@@ -213,6 +227,14 @@ __global__ void __launch_bounds__(kUpNT) k_up(const double *__restrict__ in, Lin
   // ---- step 2: parent = U0 child0 + U1 child1 inside the subtree; thread
   // (k = tid % M) keeps its operator rows in registers
   const int npair = kUpNT / M;  // (chunk, parity) pairs handled per sweep
+  double u0r[MT > 0 ? MT : 1], u1r[MT > 0 ? MT : 1];  // this thread's operator rows
+  if constexpr (MT > 0) {
+#pragma unroll
+    for (int l = 0; l < MT; l++) {
+      u0r[l] = Ul[ku * Mp + l];
+      u1r[l] = Ul[(M + ku) * Mp + l];
+    }
+  }
   int n = G;
   for (int lev = L - 2; lev >= g.lu; lev--) {
     n >>= 1;
@@ -221,21 +243,22 @@ __global__ void __launch_bounds__(kUpNT) k_up(const double *__restrict__ in, Lin
         const int c = cs >> 1, sg = cs & 1;
         const double *x0 = w0 + (2 * c * 2 + sg) * M;
         const double *x1 = w0 + ((2 * c + 1) * 2 + sg) * M;
+        const double *u0 = Ul + ku * Mp, *u1 = Ul + (M + ku) * Mp;
         double a0 = 0, a1 = 0, a2 = 0, a3 = 0;
-        if (MT > 0) {
+        if constexpr (MT > 0) {
 #pragma unroll
-          for (int l = 0; l < (MT > 0 ? MT : 1); l += 2) {
+          for (int l = 0; l < MT; l += 2) {
             a0 += u0r[l] * x0[l];
             a1 += u1r[l] * x1[l];
-            if (l + 1 < M) {
+            if (l + 1 < MT) {
               a2 += u0r[l + 1] * x0[l + 1];
               a3 += u1r[l + 1] * x1[l + 1];
             }
           }
         } else {
           for (int l = 0; l < M; l++) {
-            a0 += U[ku * M + l] * x0[l];
-            a1 += U[MM + ku * M + l] * x1[l];
+            a0 += u0[l] * x0[l];
+            a1 += u1[l] * x1[l];
           }
         }
         const double acc = (a0 + a2) + (a1 + a3);
@@ -254,15 +277,12 @@ __global__ void __launch_bounds__(kUpNT) k_up(const double *__restrict__ in, Lin
   for (int task = tid; task < np; task += kUpNT) {
     const int lev = task / M2, o = task % M2, sg = o / M, i = o % M;
     const double *x = w0 + sg * M;
+    const double *prow = Pl + (lev * M + i) * Mp;
     double acc = 0;
-    if (MT > 0 && task == tid) {
 #pragma unroll
-      for (int l = 0; l < (MT > 0 ? MT : 1); l++) acc += pr[l] * x[l];
-    } else {
-      const int k = g.lu - lev;
-      const size_t d = (size_t)j & (((size_t)1 << k) - 1);
-      const double *prow = P + ((((size_t)1 << k) - 2) + d) * MM + (size_t)i * M;
-      for (int l = 0; l < M; l++) acc += prow[l] * x[l];
+    for (int l = 0; l < (MT > 0 ? MT : 64); l++) {
+      if (MT == 0 && l >= M) break;
+      acc += prow[l] * x[l];
     }
     part[((size_t)lev * ((size_t)4 << g.lu) + j) * M2 + o] = acc;
   }
@@ -1468,19 +1488,39 @@ __global__ void __launch_bounds__(kDenseNT) k_lines_dense(const double *__restri
   const int tid = threadIdx.x, lane = tid & 63;
   const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
   const size_t l0 = (size_t)blockIdx.x * kDenseLT;
-  const int LT = kDenseLT, LTP = kDenseLTP, NE = 2 * NPP;
+  const int LTP = kDenseLTP, NE = 2 * NPP;  // NE <= 256: 32 elements per thread
   const bool contig = lm.es == 1;
-  // element e of the tile: (j, l) with j fastest when lines are contiguous,
-  // l fastest otherwise, so that neighbouring threads touch neighbouring words
-  const int tot = NE * LT;
-#pragma unroll 8
-  for (int e = tid; e < tot; e += kDenseNT) {
-    const int l = contig ? e / NE : e % LT;
-    const int j = contig ? e % NE : e / LT;
-    const size_t line = l0 + l;
-    double v = 0.0;
-    if (line < nlines && j < line_len(lm, line, N)) v = in[line_off(lm, line) + (size_t)j * lm.es];
-    sm[(j & 1) * NPP * LTP + (j >> 1) * LTP + l] = v;
+  // the tile's elements (j, l): lanes run along j when the lines are
+  // contiguous, along l (32 lines, two rows j per wave) otherwise, so that
+  // neighbouring lanes touch neighbouring words
+  // start and length of the tile's lines, computed once (line_off divides)
+  size_t *toff = (size_t *)(sm + 2 * NPP * LTP);
+  int *tlen = (int *)(toff + kDenseLT);
+  if (tid < kDenseLT) {
+    const size_t line = l0 + tid;
+    toff[tid] = line < nlines ? line_off(lm, line) : 0;
+    tlen[tid] = line < nlines ? line_len(lm, line, N) : 0;
+  }
+  __syncthreads();
+  auto in_at = [&](int j, int l) -> double {
+    return j < tlen[l] ? in[toff[l] + (size_t)j * lm.es] : 0.0;
+  };
+  // element q of this thread (q < 32 covers NE <= 256, LT = 32):
+  //   contiguous lines: line wave + 4 (q / 4), j = lane + 64 (q % 4)
+  //   otherwise:        line lane % 32,        j = 2 wave + lane / 32 + 8 q
+  // all 32 loads are issued before the first LDS store
+  auto tile_j = [&](int q) { return contig ? lane + 64 * (q & 3) : 2 * wave + (lane >> 5) + 8 * q; };
+  auto tile_l = [&](int q) { return contig ? wave + 4 * (q >> 2) : (lane & 31); };
+  double vt[32];
+#pragma unroll
+  for (int q = 0; q < 32; q++) {
+    const int j = tile_j(q);
+    vt[q] = j < NE ? in_at(j, tile_l(q)) : 0.0;
+  }
+#pragma unroll
+  for (int q = 0; q < 32; q++) {
+    const int j = tile_j(q);
+    if (j < NE) sm[(j & 1) * NPP * LTP + (j >> 1) * LTP + tile_l(q)] = vt[q];
   }
   __syncthreads();
 
@@ -1509,10 +1549,12 @@ __global__ void __launch_bounds__(kDenseNT) k_lines_dense(const double *__restri
       if (h == 1 && R == Rp) break;
       rtile[q][h] = R;
       const int nk = (NPP - 16 * R) >> 2;  // k-steps from the diagonal tile on
-      const double *Kr = K + ((size_t)p * NPP + 16 * R + (lane & 15)) * NPP + 16 * R + (lane >> 4);
+      // A fragments of row tile R from its diagonal k-step on: 64 consecutive
+      // doubles each
+      const double *Kr = K + (((size_t)p * NR + R) * (NPP >> 2) + 4 * R) * 64 + lane;
       double av[32];
 #pragma unroll
-      for (int k = 0; k < 32; k++) av[k] = k < nk ? Kr[4 * k] : 0.0;
+      for (int k = 0; k < 32; k++) av[k] = k < nk ? Kr[64 * k] : 0.0;
       d4 D0 = {0.0, 0.0, 0.0, 0.0}, D1 = {0.0, 0.0, 0.0, 0.0};
 #pragma unroll
       for (int k = 0; k < 32; k++) {
@@ -1542,17 +1584,20 @@ __global__ void __launch_bounds__(kDenseNT) k_lines_dense(const double *__restri
     }
   }
   __syncthreads();
-#pragma unroll 8
-  for (int e = tid; e < tot; e += kDenseNT) {
-    const int l = contig ? e / NE : e % LT;
-    const int j = contig ? e % NE : e / LT;
-    const size_t line = l0 + l;
-    if (line < nlines && j < line_len(lm, line, N))
-      out[line_off(lm, line) + (size_t)j * lm.es] = sm[(j & 1) * NPP * LTP + (j >> 1) * LTP + l];
+  auto out_at = [&](int j, int l) {
+    if (j < tlen[l]) out[toff[l] + (size_t)j * lm.es] = sm[(j & 1) * NPP * LTP + (j >> 1) * LTP + l];
+  };
+#pragma unroll
+  for (int q = 0; q < 32; q++) {
+    const int j = tile_j(q);
+    if (j < NE) out_at(j, tile_l(q));
   }
 }
 
-static size_t dense_shared_bytes(int NPP) { return 8 * (size_t)2 * NPP * kDenseLTP; }
+// the tile [2][NPP][LTP], then the lines' offsets (size_t) and lengths (int)
+static size_t dense_shared_bytes(int NPP) {
+  return 8 * (size_t)2 * NPP * kDenseLTP + kDenseLT * (sizeof(size_t) + sizeof(int));
+}
 
 // Batched lines through a direct plan: dense MFMA products when the plan has
 // them and the batch is large enough, else one row per thread.

@@ -146,7 +146,10 @@ __global__ void k_fill_dn(const double *a, double *dn, int n) {
 // batched lines: out[2r+p] = sum_c K[p][r][c] u[2c+p], r, c < NPP, with the
 // final scaling (2/pi and b_0/2 for L2C; (i+1/2) for C2L) and the C2L input
 // weights v_j = j u_j (v_0 = u_0) folded in; zero below the diagonal and past
-// the length.  Grid (NPP, 2), one row per work-group.
+// the length.  Stored in MFMA A-operand order: the 16 x 4 tile (row tile
+// R = r/16, k-step c/4) of parity p is 64 consecutive doubles, lane
+// (r % 16) + 16 (c % 4), at ((p NR + R) (NPP/4) + c/4) 64, NR = NPP/16.
+// Grid (NPP, 2), one row per work-group.
 __global__ void k_build_dense(double *K, const double *__restrict__ a,
                               const double *__restrict__ an, const double *__restrict__ dn,
                               int N, int NPP, int dir) {
@@ -166,7 +169,8 @@ __global__ void k_build_dense(double *K, const double *__restrict__ a,
         v *= (double)i + 0.5;
       }
     }
-    K[((size_t)p * NPP + r) * NPP + c] = v;
+    const size_t tile = ((size_t)p * (NPP >> 4) + (r >> 4)) * (NPP >> 2) + (c >> 2);
+    K[tile * 64 + (r & 15) + 16 * (c & 3)] = v;
   }
 }
 
@@ -413,7 +417,8 @@ static bool choose_cuts(Geo &g) {
   int gu = 4, gd = g.fast ? 4 : 3;
   if (const char *e = getenv("L2C_GU")) gu = atoi(e);
   if (const char *e = getenv("L2C_GD")) gd = atoi(e);
-  const size_t budget = 64 * 1024, dbudget = (g.fast ? 80 : 64) * 1024;
+  // k_up holds its composite operators in LDS (lu of them): up to 80 KiB
+  const size_t budget = 80 * 1024, dbudget = (g.fast ? 80 : 64) * 1024;
   g.gu = std::max(0, std::min(std::min(gu, 4), Lm1));  // k_up: at most 16 leaf chunks
   g.gd = std::max(0, std::min(std::min(gd, g.fast ? 4 : Lm1), Lm1));
   for (;;) {
