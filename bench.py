@@ -340,6 +340,8 @@ def main():
                     help="also time this maxs (the published figure's leaf size); 0 = skip")
     ap.add_argument("--pcie-steps", type=int, default=20,
                     help="host-pointer execute() calls timed for the PCIe-inclusive rate; 0 = skip")
+    ap.add_argument("--split-steps", type=int, default=200,
+                    help="steps timed with precision form 1 (split M2L matrices); 0 = skip")
     ap.add_argument("--batch", type=int, default=4096,
                     help="transforms in the uneven batch (log-uniform n in [64, 8192]); 0 = skip")
     ap.add_argument("--copy-mb", type=int, default=0,
@@ -437,6 +439,36 @@ def main():
                 "traffic": pmc_traffic(workload, dom), "algorithmic_bytes": alg,
                 "avg_launch_us": round(avg_us, 3)}
 
+    # precision form 1 (split M2L matrices) on the same plan and input,
+    # reported beside the FP64 value
+    split = None
+    if args.split_steps > 0:
+        plan.set_precision(1)
+        for _ in range(args.warmup):
+            plan.execute_device(din.ptr, dout.ptr, d)
+        plan.profile(args.prof_every)
+        plan.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(args.split_steps):
+            plan.execute_device(din.ptr, dout.ptr, d)
+        plan.synchronize()
+        el_s = time.perf_counter() - t0
+        prof_s = plan.profile_read()
+        plan.profile(False)
+        gs = dout.download()
+        plan.set_precision(0)
+        e_s = float(np.max(np.abs(gs - o)) / np.max(np.abs(o)))
+        e_sd = float(np.max(np.abs(gs[rows] - ref)) / scale)
+        split = {"form": "split M2L matrices (per-position FP64/FP32/BF16/dropped), FP64 arithmetic",
+                 "value": round(args.split_steps / el_s, 2), "unit": "transforms/s",
+                 "ms_per_step": round(1e3 * el_s / args.split_steps, 5),
+                 "kernels_us": {k: round(1e3 * ms / max(c, 1), 3) for k, (ms, c) in prof_s.items()},
+                 "max_rel_err": {"vs_oracle_fmm": e_s, "vs_dd_direct_rows": e_sd}}
+        log(f"[bench] split form: {split}")
+        if not (e_s <= tol and e_sd <= tol):
+            log("[bench] split form error above tolerance")
+            split["failed"] = True
+
     # the drop-in execute() on host arrays: staged through pinned buffers,
     # PCIe included (reported beside the value, never as the value)
     pcie = None
@@ -505,12 +537,15 @@ def main():
         "roofline": roofline,
         "cpu_baseline": cpu,
         "host_pointer_transforms_s": pcie,
+        "split_form": split,
         "batched": batched,
         "at_published_leaf_size": cmp_line,
     }
     print(json.dumps(line), flush=True)
     bad = not (err_fmm <= tol and err_direct <= tol)
     if cmp_line is not None and not cmp_line["max_rel_err_vs_oracle"] <= tol:
+        bad = True
+    if split is not None and split.get("failed"):
         bad = True
     if batched is not None and not batched["max_rel_err_vs_oracle"] <= 1e-14 * math.log2(8192):
         bad = True

@@ -352,10 +352,50 @@ __device__ __forceinline__ void m2l_block_compute(const double *Al, const double
   }
 }
 
+// Split-form M2L block (precision form 1): record slots in tier order (FP64,
+// FP32, BF16), decoded to FP64 into the padded LDS block at pos[slot]; the
+// dropped positions (slots past the three tiers) are written as zeros.
+struct SplitView {
+  const unsigned char *rec;
+  const unsigned short *pos;
+  int n64, n32, n16, rec_bytes;
+};
+template <int NTH>
+__device__ __forceinline__ void split_unpack(const SplitView &sv, size_t block, int n3, double *Al,
+                                             int t) {
+  // Every slot reads the aligned 8-byte word holding its value (FP64: the
+  // value; FP32: one half; BF16: one quarter), so the loads of a batch carry
+  // no branch and all issue before the first LDS store.
+  const unsigned char *rb = sv.rec + block * sv.rec_bytes;
+  const int n2 = sv.n64 + sv.n32, nt = n2 + sv.n16;
+  for (int i0 = 0; i0 < n3; i0 += 8 * NTH) {
+    unsigned long long wv[8];
+    int d[8], o[8];
+#pragma unroll
+    for (int q = 0; q < 8; q++) {
+      const int i = i0 + q * NTH + t;
+      d[q] = i < n3 ? (int)sv.pos[i] : -1;
+      o[q] = i < sv.n64 ? 8 * i : i < n2 ? 8 * sv.n64 + 4 * (i - sv.n64) : 8 * sv.n64 + 4 * sv.n32 + 2 * (i - n2);
+      const int oc = i < nt ? o[q] : 0;  // dropped slots read the record's first word
+      wv[q] = *(const unsigned long long *)(rb + (oc & ~7));
+    }
+#pragma unroll
+    for (int q = 0; q < 8; q++) {
+      const int i = i0 + q * NTH + t;
+      const int sh = 8 * (o[q] & 7);
+      const double v64 = __longlong_as_double((long long)wv[q]);
+      const double v32 = (double)__uint_as_float((unsigned)(wv[q] >> sh));
+      const double v16 = (double)__uint_as_float(((unsigned)(wv[q] >> sh) & 0xFFFFu) << 16);
+      const double v = i < sv.n64 ? v64 : i < n2 ? v32 : i < nt ? v16 : 0.0;
+      if (d[q] >= 0) Al[d[q]] = v;
+    }
+  }
+}
+
 template <int MT>
 __global__ void __launch_bounds__(kM2lNT) k_m2l(Geo g, const double *__restrict__ A,
                                                 double *__restrict__ wsall, size_t wsstride,
-                                                int bpw, size_t nlines, int lpb) {
+                                                int bpw, size_t nlines, int lpb, SplitView sv) {
   extern __shared__ double sm[];
   const int M = MT > 0 ? MT : g.M, M2 = 2 * M, MM = M * M, Mp = M + 1;
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
@@ -377,19 +417,23 @@ __global__ void __launch_bounds__(kM2lNT) k_m2l(Geo g, const double *__restrict_
     while ((int)total_blocks(lev + 1) <= B) lev++;
     const int b = B - (int)total_blocks(lev);
     double *Al = sm, *wl = sm + 3 * M * Mp, *red = sm + WL;
-    const double *Ab = A + (size_t)(3 * total_blocks(lev) + 3 * b) * MM;
-    double va[8];
+    if (sv.rec) {
+      split_unpack<kM2lNT>(sv, (size_t)B, 3 * MM, Al, tid);
+    } else {
+      const double *Ab = A + (size_t)(3 * total_blocks(lev) + 3 * b) * MM;
+      double va[8];
 #pragma unroll
-    for (int q = 0; q < 8; q++) {
-      const int e = q * kM2lNT + tid;
-      va[q] = e < 3 * MM ? Ab[e] : 0.0;
-    }
+      for (int q = 0; q < 8; q++) {
+        const int e = q * kM2lNT + tid;
+        va[q] = e < 3 * MM ? Ab[e] : 0.0;
+      }
 #pragma unroll
-    for (int q = 0; q < 8; q++) {
-      const int e = q * kM2lNT + tid;
-      if (e < 3 * MM) Al[(e / M) * Mp + e % M] = va[q];
+      for (int q = 0; q < 8; q++) {
+        const int e = q * kM2lNT + tid;
+        if (e < 3 * MM) Al[(e / M) * Mp + e % M] = va[q];
+      }
+      for (int e = 8 * kM2lNT + tid; e < 3 * MM; e += kM2lNT) Al[(e / M) * Mp + e % M] = Ab[e];
     }
-    for (int e = 8 * kM2lNT + tid; e < 3 * MM; e += kM2lNT) Al[(e / M) * Mp + e % M] = Ab[e];
     const int nw = 2 * M2;  // multipole doubles of chunks 2b+2, 2b+3
     for (int li = 0; li < nl; li++) {
       double *ws = wsall + (line0 + li) * wsstride;
@@ -445,27 +489,31 @@ __global__ void __launch_bounds__(kM2lNT) k_m2l(Geo g, const double *__restrict_
   while ((int)total_blocks(lev + 1) <= B) lev++;
   const int b = B - (int)total_blocks(lev);
   double *Al = sm + wave * WL, *wl = Al + 3 * M * Mp;
-  const double *Ab = A + (size_t)(3 * total_blocks(lev) + 3 * b) * MM;
   const size_t wofs = lay.w + (slot_base(lev) + 2 * (size_t)b + 2) * M2;  // chunks 2b+2, 2b+3
   const size_t oofs = lay.m2l + (slot_base(lev) + 2 * (size_t)b) * M2;
-  constexpr int QA = MT > 0 ? (3 * MT * MT + 63) / 64 : 16;
-  double va[QA];
-#pragma unroll
-  for (int q = 0; q < QA; q++) {
-    const int e = q * 64 + lane;
-    va[q] = e < 3 * MM ? Ab[e] : 0.0;
-  }
   // multipoles of the first line; later lines are fetched one line ahead
   const double *wb = wsall + line0 * wsstride + wofs;
   double w0v = lane < 2 * M2 ? wb[lane] : 0.0;  // 2*M2 <= 192
   double w1v = lane + 64 < 2 * M2 ? wb[lane + 64] : 0.0;
   double w2v = lane + 128 < 2 * M2 ? wb[lane + 128] : 0.0;
+  if (sv.rec) {
+    split_unpack<64>(sv, (size_t)B, 3 * MM, Al, lane);
+  } else {
+    const double *Ab = A + (size_t)(3 * total_blocks(lev) + 3 * b) * MM;
+    constexpr int QA = MT > 0 ? (3 * MT * MT + 63) / 64 : 16;
+    double va[QA];
 #pragma unroll
-  for (int q = 0; q < QA; q++) {
-    const int e = q * 64 + lane;
-    if (e < 3 * MM) Al[(e / M) * Mp + e % M] = va[q];
+    for (int q = 0; q < QA; q++) {
+      const int e = q * 64 + lane;
+      va[q] = e < 3 * MM ? Ab[e] : 0.0;
+    }
+#pragma unroll
+    for (int q = 0; q < QA; q++) {
+      const int e = q * 64 + lane;
+      if (e < 3 * MM) Al[(e / M) * Mp + e % M] = va[q];
+    }
+    for (int e = QA * 64 + lane; e < 3 * MM; e += 64) Al[(e / M) * Mp + e % M] = Ab[e];
   }
-  for (int e = QA * 64 + lane; e < 3 * MM; e += 64) Al[(e / M) * Mp + e % M] = Ab[e];
   for (int li = 0; li < nl; li++) {
     if (lane < 2 * M2) wl[lane] = w0v;
     if (lane + 64 < 2 * M2) wl[lane + 64] = w1v;
@@ -1684,8 +1732,11 @@ static int launch_fmm_m(fmm_plan_s *p, const double *d_in, double *d_out, size_t
     while (lpb < 8 && (size_t)gx * ((nlines + 2 * lpb - 1) / (2 * lpb)) >= 2048) lpb *= 2;
     dim3 gm(gx, (unsigned)((nlines + lpb - 1) / lpb));
     prof_begin(p->prof, K_M2L, st);
+    const DevTables::Split &sp = p->t.sp[dir];
+    const SplitView sv{p->precision == 1 ? sp.rec : nullptr, sp.pos, sp.n64, sp.n32, sp.n16,
+                       sp.rec_bytes};
     k_m2l<MT><<<gm, kM2lNT, m2l_shared_bytes(g), st>>>(g, p->t.A[dir], ws, wsstride, bpw,
-                                                      nlines, lpb);
+                                                      nlines, lpb, sv);
     prof_end(p->prof, st);
   }
   if (g.fast) {

@@ -122,7 +122,18 @@ struct DevTables {
   // band head per direction: Lambda(k) (L2C) or dn(k) (C2L) for k < min(2s,
   // table length), zero up to 2s + 16
   double *head[2] = {nullptr, nullptr};
-  double *Asl[2] = {nullptr, nullptr};  // split-precision M2L slices
+  // Split form of the M2L matrices (precision form 1), per direction: every
+  // block (its three M x M submatrices, 3 M^2 coefficients) is one record of
+  // rec_bytes: the coefficients of storage tier FP64 (n64), then FP32 (n32),
+  // then BF16 (n16); the remaining coefficient positions are dropped (zero).
+  // pos[i] is the padded LDS index (row (M+1) + column) of record slot i, the
+  // dropped positions last.  Tiers are per coefficient position, chosen at
+  // plan time from the largest magnitude relative to its block's largest.
+  struct Split {
+    unsigned char *rec = nullptr;
+    unsigned short *pos = nullptr;
+    int n64 = 0, n32 = 0, n16 = 0, rec_bytes = 0;
+  } sp[2];
 };
 
 // Per-launch HIP-event timing of the execute kernels (bench support).
@@ -164,7 +175,7 @@ struct fmm_plan_s {
   size_t N = 0, maxs = 0, M = 0, direction = 0, lagrange = 0, precompute = 0;
   size_t Nn = 0, L = 0, s = 0, Nc = 0, Nb = 0;
   int device = 0;
-  int precision = 0;  // 0 plain FP64, 1 split precision
+  int precision = 0;  // 0 plain FP64, 1 split form of the M2L matrices
   l2c::Geo geo{};
   l2c::DevTables t;
   direct_plan_s *dplan = nullptr;  // L < 1: direct path only

@@ -289,6 +289,136 @@ static void dfree(void *p) {
   if (p) (void)hipFree(p);
 }
 
+// ---------------------------------------------------------------------------
+// Split form of the M2L matrices (precision form 1)
+
+// Largest |coefficient| of each block (3 M^2 values); one wave per block.
+__global__ void k_block_absmax(const double *__restrict__ A, double *bmax, int n3) {
+  const size_t b = blockIdx.x;
+  double m = 0.0;
+  for (int e = threadIdx.x; e < n3; e += 64) m = fmax(m, fabs(A[b * n3 + e]));
+  for (int o = 32; o > 0; o >>= 1) m = fmax(m, __shfl_xor(m, o));
+  if (threadIdx.x == 0) bmax[b] = m;
+}
+
+// Per coefficient position e: the largest |A_e| / (block max) over the blocks
+// b = blockIdx.y, blockIdx.y + gridDim.y, ...; partial maxima per grid row.
+__global__ void k_pos_ratio(const double *__restrict__ A, const double *__restrict__ bmax,
+                            double *part, int nb, int n3) {
+  const int e = blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= n3) return;
+  double m = 0.0;
+  for (int b = blockIdx.y; b < nb; b += gridDim.y) {
+    const double d = bmax[b];
+    if (d > 0) m = fmax(m, fabs(A[(size_t)b * n3 + e]) / d);
+  }
+  part[(size_t)blockIdx.y * n3 + e] = m;
+}
+
+__device__ inline unsigned short bf16_rne(float f) {
+  unsigned u = __float_as_uint(f);
+  u += 0x7FFFu + ((u >> 16) & 1u);
+  return (unsigned short)(u >> 16);
+}
+
+// One record per block: slot i holds coefficient src[i] of the block in its
+// tier's type.
+__global__ void k_pack_split(const double *__restrict__ A, unsigned char *rec,
+                             const unsigned short *__restrict__ src, int n64, int n32, int n16,
+                             int n3, int rec_bytes) {
+  const size_t b = blockIdx.x;
+  const double *Ab = A + b * n3;
+  unsigned char *r = rec + b * rec_bytes;
+  double *r64 = (double *)r;
+  float *r32 = (float *)(r + 8 * (size_t)n64);
+  unsigned short *r16 = (unsigned short *)(r + 8 * (size_t)n64 + 4 * (size_t)n32);
+  for (int i = threadIdx.x; i < n64 + n32 + n16; i += blockDim.x) {
+    const double v = Ab[src[i]];
+    if (i < n64)
+      r64[i] = v;
+    else if (i < n64 + n32)
+      r32[i - n64] = (float)v;
+    else
+      r16[i - n64 - n32] = bf16_rne((float)v);
+  }
+}
+
+// Storage tier of a coefficient position from its largest magnitude relative
+// to its block's largest, so that the stored value's error stays below
+// 2^-53 of the block's largest coefficient: FP32 (24-bit significand) below
+// 2^-29, BF16 (8-bit) below 2^-44, dropped below 2^-53.
+static int split_tier(double ratio) {
+  if (ratio > std::ldexp(1.0, -29)) return 0;
+  if (ratio > std::ldexp(1.0, -44)) return 1;
+  if (ratio > std::ldexp(1.0, -53)) return 2;
+  return 3;
+}
+
+static int build_split(fmm_plan_s *p, int dir) {
+  hipStream_t st = p->stream;
+  const int M = (int)p->M, Mp = M + 1, n3 = 3 * M * M;
+  const int nb = (int)p->Nb;
+  const double *A = p->t.A[dir];
+  double *bmax = nullptr, *part = nullptr;
+  const int rows = std::min(nb, 256);
+  if (dmalloc(&bmax, (size_t)nb, nullptr) || dmalloc(&part, (size_t)rows * n3, nullptr)) {
+    dfree(bmax);
+    return -1;
+  }
+  k_block_absmax<<<nb, 64, 0, st>>>(A, bmax, n3);
+  k_pos_ratio<<<dim3((n3 + 255) / 256, rows), 256, 0, st>>>(A, bmax, part, nb, n3);
+  std::vector<double> hp((size_t)rows * n3);
+  const hipError_t e1 = hipMemcpyAsync(hp.data(), part, hp.size() * 8, hipMemcpyDeviceToHost, st);
+  const hipError_t e2 = hipStreamSynchronize(st);
+  dfree(bmax);
+  dfree(part);
+  if (e1 != hipSuccess || e2 != hipSuccess) {
+    set_error("build_split: ratio reduction failed");
+    return -1;
+  }
+  std::vector<int> tier(n3);
+  int cnt[4] = {0, 0, 0, 0};
+  for (int e = 0; e < n3; e++) {
+    double r = 0;
+    for (int y = 0; y < rows; y++) r = std::max(r, hp[(size_t)y * n3 + e]);
+    tier[e] = split_tier(r);
+    cnt[tier[e]]++;
+  }
+  std::vector<unsigned short> src, pos;
+  for (int t = 0; t < 4; t++)
+    for (int e = 0; e < n3; e++)
+      if (tier[e] == t) {
+        src.push_back((unsigned short)e);
+        pos.push_back((unsigned short)((e / M) * Mp + e % M));
+      }
+  DevTables::Split &sp = p->t.sp[dir];
+  sp.n64 = cnt[0];
+  sp.n32 = cnt[1];
+  sp.n16 = cnt[2];
+  sp.rec_bytes = (8 * cnt[0] + 4 * cnt[1] + 2 * cnt[2] + 15) / 16 * 16;
+  unsigned short *d_src = nullptr;
+  auto fail = [&]() {
+    dfree(d_src);
+    dfree(sp.rec);
+    dfree(sp.pos);
+    sp.rec = nullptr;
+    sp.pos = nullptr;
+    return -1;
+  };
+  if (dmalloc(&sp.rec, (size_t)nb * sp.rec_bytes, &p->hbm_bytes) ||
+      dmalloc(&sp.pos, (size_t)n3, &p->hbm_bytes) || dmalloc(&d_src, (size_t)n3, nullptr))
+    return fail();
+  L2C_HIP(hipMemcpyAsync(sp.pos, pos.data(), n3 * sizeof(unsigned short), hipMemcpyHostToDevice, st), fail());
+  L2C_HIP(hipMemcpyAsync(d_src, src.data(), n3 * sizeof(unsigned short), hipMemcpyHostToDevice, st), fail());
+  L2C_HIP(hipMemsetAsync(sp.rec, 0, (size_t)nb * sp.rec_bytes, st), fail());
+  k_pack_split<<<nb, 256, 0, st>>>(A, sp.rec, d_src, sp.n64, sp.n32, sp.n16, n3, sp.rec_bytes);
+  L2C_HIP(hipGetLastError(), fail());
+  L2C_HIP(hipStreamSynchronize(st), fail());
+  dfree(d_src);
+  return 0;
+}
+
+
 // Lambda tables on the device: a (N), and an (N), dn ((N+1)/2) when C2L.
 static int build_lambda_tables(size_t N, bool c2l, double **a, double **an, double **dn,
                                hipStream_t st, size_t *acct) {
@@ -381,8 +511,10 @@ static void destroy_fmm(fmm_plan_s *p) {
   dfree(p->t.dn);
   dfree(p->t.A[0]);
   dfree(p->t.A[1]);
-  dfree(p->t.Asl[0]);
-  dfree(p->t.Asl[1]);
+  for (int d = 0; d < 2; d++) {
+    dfree(p->t.sp[d].rec);
+    dfree(p->t.sp[d].pos);
+  }
   dfree(p->t.T);
   dfree(p->t.U);
   dfree(p->t.D);
@@ -855,9 +987,15 @@ int l2c_synchronize(fmm_plan *p) {
   return 0;
 }
 int l2c_set_precision(fmm_plan *p, int form) {
-  if (!p || form != 0) {
-    set_error("l2c_set_precision: only the plain FP64 form (0) is available");
+  clear_error();
+  if (!p || (form != 0 && form != 1)) {
+    set_error("l2c_set_precision: form must be 0 (plain FP64) or 1 (split M2L matrices)");
     return -1;
+  }
+  if (form == 1 && p->L >= 1) {
+    L2C_HIP(hipStreamSynchronize(p->stream), -1);
+    for (int d = 0; d < 2; d++)
+      if (p->t.A[d] && !p->t.sp[d].rec && build_split(p, d)) return -1;
   }
   p->precision = form;
   return 0;

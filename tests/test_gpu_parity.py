@@ -142,3 +142,43 @@ def test_2d(amd, orc, axis, N0, N1):
         g = amd.FMMPlan2D(N0, N1, axis, 64).execute(x, d)
         o = orc.FMMPlan2D(N0, N1, axis, 64).execute(x, d)
         assert relerr(g, o) <= tol(max(N0, N1))
+
+
+@pytest.mark.parametrize("N,maxs", [(4096, 64), (65536, 64), (12345, 64), (8192, 32), (3000, 32),
+                                    (1 << 18, 64)])
+def test_split_form(amd, orc, N, maxs):
+    """Precision form 1 (M2L matrices stored per coefficient position as FP64,
+    FP32, BF16 or dropped, from their magnitude relative to the block's
+    largest): same tolerance against the oracle, and within a few ulps of
+    form 0; rows 0-255 and sampled rows against the double-double direct sum."""
+    announce(f"split form N={N} maxs={maxs}")
+    p = amd.FMMPlan(N, maxs, 18, amd.BOTH)
+    for d in (0, 1):
+        u = amd.coefficients(N, seed=300 + d, decay=0.0)
+        g0 = p.execute(u, d)
+        p.set_precision(1)
+        g1 = p.execute(u, d)
+        p.set_precision(0)
+        o = orc.FMMPlan(N, maxs, 18, d).execute(u, d)
+        e1, e01 = relerr(g1, o), relerr(g1, g0)
+        print(f"  dir {d}: form 1 vs oracle {e1:.3e}, vs form 0 {e01:.3e}")
+        assert e1 <= tol(N)
+        assert e01 <= 1e-14
+        rows = np.unique(np.concatenate([np.arange(256),
+                                         np.random.default_rng(N).integers(0, N, 256)]))
+        ref = orc.direct_dd_rows(u, rows, d)
+        assert float(np.max(np.abs(g1[rows] - ref))) / float(np.max(np.abs(o))) <= tol(N)
+
+
+def test_split_form_nodal(amd, orc):
+    """Nodal M2L matrices (lagrange=1) hold samples that do not decay: the
+    split form keeps them in FP64 and the transform is unchanged."""
+    announce("split form nodal")
+    N = 5000
+    u = amd.coefficients(N, seed=17)
+    p = amd.FMMPlan(N, 64, 18, 0, 1)
+    g0 = p.execute(u, 0)
+    p.set_precision(1)
+    g1 = p.execute(u, 0)
+    assert relerr(g1, g0) <= 1e-15
+    assert relerr(g1, orc.FMMPlan(N, 64, 18, 0, 1).execute(u, 0)) <= tol(N)
