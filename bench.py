@@ -316,9 +316,10 @@ def bench_field(amd, args):
 def main():
     ap = argparse.ArgumentParser(description=__doc__, formatter_class=argparse.RawDescriptionHelpFormatter)
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--workload", choices=("fmm", "field"), default="fmm",
-                    help="fmm: the N=2^20 transform and the uneven batch (default); field: the "
-                         "256^3 strided field with 2^18 vectors mixed in")
+    ap.add_argument("--workload", choices=("fmm", "batch", "field"), default="fmm",
+                    help="fmm: the N=2^20 transform and the uneven batch (default); batch: the "
+                         "uneven batch alone; field: the 256^3 strided field with 2^18 vectors "
+                         "mixed in")
     ap.add_argument("--field-n", type=int, default=256)
     ap.add_argument("--field-vectors", type=int, default=64)
     ap.add_argument("--steps", type=int, default=400)
@@ -373,6 +374,21 @@ def main():
     amd = load_pkg()
     if args.workload == "field":
         bench_field(amd, args)
+        return
+    if args.workload == "batch":
+        sys.path.insert(0, ROOT)
+        from oracle import orc
+        b = bench_batch(amd, orc, args)
+        line = {"metric": "batched L2C transforms/s, 4096 transforms of log-uniform N in [64, 8192]",
+                "value": b["value"], "unit": "transforms/s", "n_gpus": 1,
+                "steps": max(10, args.steps // 4), "ms_per_step": b["ms_per_batch"],
+                "higher_is_better": True, "dtype": "f64",
+                "data": "synthetic (SplitMix64 lengths seed 2024, inputs seed 77, U(-1,1)/(k+1))",
+                "config": {"workload": f"batch{args.batch}_loguniform_64_8192", "M": args.M},
+                "batched": b}
+        print(json.dumps(line), flush=True)
+        if not b["max_rel_err_vs_oracle"] <= 1e-14 * math.log2(8192):
+            sys.exit(1)
         return
     N, maxs, M, d = args.N, args.maxs, args.M, args.direction
     u = amd.coefficients(N, seed=1, decay=args.decay)
