@@ -767,14 +767,18 @@ __device__ __forceinline__ void subtree_locals(const Geo &g, int Rr, int tid, in
   }
   lds_sync();
   for (int o = tid; o < M2; o += NT) {
-    // fixed trip count so the partial-sum reads issue together (ld < 28)
+    // the chain terms in level order; four reads in flight per step
     double v = mres[g.ld * M2 + o];
-    const int lmax = g.ld > 0 ? g.ld - 1 : 0;
-#pragma unroll
-    for (int lev = 0; lev < 28; lev++) {
-      const double q = qp[std::min(lev, lmax) * M2 + o];
-      v += lev < g.ld ? q : 0.0;
+    int lev = 0;
+    for (; lev + 4 <= g.ld; lev += 4) {
+      const double q0 = qp[lev * M2 + o], q1 = qp[(lev + 1) * M2 + o];
+      const double q2 = qp[(lev + 2) * M2 + o], q3 = qp[(lev + 3) * M2 + o];
+      v += q0;
+      v += q1;
+      v += q2;
+      v += q3;
     }
+    for (; lev < g.ld; lev++) v += qp[lev * M2 + o];
     cres[o] = v;
   }
   lds_sync();
