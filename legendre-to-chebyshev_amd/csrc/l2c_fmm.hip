@@ -713,11 +713,14 @@ __device__ __forceinline__ size_t m2l_elem(const Geo &g, int Rr, int e, int M2, 
 
 // Row k of the composite operator Q(ld - lev, Rr mod 2^(ld - lev)) for chain
 // task (lev, k) = (task / M, task % M).
+// Row k of the chain operator of task (lev, k).  The plan stores each Q
+// matrix column-major, so the row's elements are M apart (qrow[l * M]) and
+// the M threads of one level, rows k = 0..M-1, read consecutive words.
 __device__ __forceinline__ const double *qrow_ptr(const double *Qg, const Geo &g, int Rr, int task,
                                                   int M) {
   const int lev = task / M, k = task - lev * M, kk = g.ld - lev;
   const size_t d = (size_t)Rr & (((size_t)1 << kk) - 1);
-  return Qg + ((((size_t)1 << kk) - 2) + d) * (size_t)(M * M) + (size_t)k * M;
+  return Qg + ((((size_t)1 << kk) - 2) + d) * (size_t)(M * M) + (size_t)k;
 }
 
 // Local expansions (L2L, src/C/leg2cheb.c:1437-1449) of the work-group's
@@ -759,8 +762,8 @@ __device__ __forceinline__ void subtree_locals(const Geo &g, int Rr, int tid, in
     const double *qrow = qrow_ptr(Qg, g, Rr, task, M);
     double o0 = 0, o1 = 0;
     for (int l = 0; l < M; l++) {
-      o0 += qrow[l] * mv[l];
-      o1 += qrow[l] * mv[M + l];
+      o0 += qrow[l * M] * mv[l];
+      o1 += qrow[l * M] * mv[M + l];
     }
     qp[lev * M2 + k] = o0;
     qp[lev * M2 + M + k] = o1;
@@ -944,7 +947,7 @@ __global__ void __launch_bounds__(kDownNT) __attribute__((amdgpu_waves_per_eu(2)
   if (MT > 0 && nq > 0) {
     const double *qrow = qrow_ptr(Qg, g, Rr, tid < nq ? tid : 0, M);
 #pragma unroll
-    for (int l = 0; l < (MT > 0 ? MT : 1); l++) qr[l] = qrow[l];
+    for (int l = 0; l < (MT > 0 ? MT : 1); l++) qr[l] = qrow[l * M];
   }
 #pragma unroll
   for (int q = 0; q < QW; q++) {
@@ -1272,7 +1275,7 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) k
     if (MT > 0 && nq > 0) {
       const double *qrow = qrow_ptr(Qg, g, Rr, tid < nq ? tid : 0, M);
 #pragma unroll
-      for (int l = 0; l < (MT > 0 ? MT : 1); l++) qr[l] = qrow[l];
+      for (int l = 0; l < (MT > 0 ? MT : 1); l++) qr[l] = qrow[l * M];
     }
 #pragma unroll
     for (int q = 0; q < QD; q++) {

@@ -779,8 +779,14 @@ fmm_plan_s *create_fmm_impl(size_t N, size_t maxs, size_t M, size_t direction, s
     destroy_fmm(p);
     return nullptr;
   }
+  // the device copy of every Q matrix is column-major: the chain threads of
+  // one level read rows k = 0..M-1 of it, which then lie in consecutive words
+  std::vector<double> Qc(Q.size());
+  for (size_t m = 0; m < nQ; m++)
+    for (int k = 0; k < Mi; k++)
+      for (int l = 0; l < Mi; l++) Qc[m * MM + (size_t)l * M + k] = Q[m * MM + (size_t)k * M + l];
   if (nQ)
-    L2C_HIP(hipMemcpyAsync(p->t.Q, Q.data(), nQ * MM * 8, hipMemcpyHostToDevice, st), (destroy_fmm(p), nullptr));
+    L2C_HIP(hipMemcpyAsync(p->t.Q, Qc.data(), nQ * MM * 8, hipMemcpyHostToDevice, st), (destroy_fmm(p), nullptr));
   L2C_HIP(hipMemcpyAsync(p->t.T, T.data(), T.size() * 8, hipMemcpyHostToDevice, st), (destroy_fmm(p), nullptr));
   L2C_HIP(hipMemcpyAsync(p->t.U, U.data(), U.size() * 8, hipMemcpyHostToDevice, st), (destroy_fmm(p), nullptr));
   L2C_HIP(hipMemcpyAsync(p->t.D, D.data(), D.size() * 8, hipMemcpyHostToDevice, st), (destroy_fmm(p), nullptr));
