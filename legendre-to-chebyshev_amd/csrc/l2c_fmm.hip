@@ -198,13 +198,16 @@ __global__ void __launch_bounds__(kUpNT) k_up(const double *__restrict__ in, Lin
     const double *tcol = T + (size_t)sg * s * M + kb;
     d4 acc = {0, 0, 0, 0};
     int m0 = 0;
-    if (task == wave) {  // B prefetched for rows < 64
+    if (task == wave) {  // B prefetched for rows < 64; two independent chains
+      d4 acc1 = {0, 0, 0, 0};
 #pragma unroll
-      for (int q = 0; q < QT; q++) {
+      for (int q = 0; q < QT; q += 2) {
         const int m = 4 * q + ka;
-        const double a = m < s ? xrow[m] : 0.0;
-        acc = __builtin_amdgcn_mfma_f64_16x16x4f64(a, tb[q], acc, 0, 0, 0);
+        const double a0 = m < s ? xrow[m] : 0.0, a1 = m + 4 < s ? xrow[m + 4] : 0.0;
+        acc = __builtin_amdgcn_mfma_f64_16x16x4f64(a0, tb[q], acc, 0, 0, 0);
+        acc1 = __builtin_amdgcn_mfma_f64_16x16x4f64(a1, tb[q + 1], acc1, 0, 0, 0);
       }
+      acc += acc1;
       m0 = 4 * QT;
     }
     for (; m0 < s; m0 += 4) {
@@ -1598,30 +1601,40 @@ __global__ void __launch_bounds__(kDenseNT) k_lines_dense(const double *__restri
     const int p = jp & 1, Rp = jp >> 1;
     rpar[q] = p;
     const double *Bp = sm + p * NPP * LTP + (lane >> 4) * LTP + (lane & 15);
+    // the pair's two row tiles run together: four independent MFMA chains
+    // (row tile x column tile); the second tile is the shorter one
+    const int R0 = Rp, R1 = NR - 1 - Rp;
+    const bool two = R1 != R0;
+    rtile[q][0] = R0;
+    rtile[q][1] = two ? R1 : -1;
+    const int nk0 = (NPP - 16 * R0) >> 2, nk1 = two ? (NPP - 16 * R1) >> 2 : 0;
+    // A fragments of a row tile from its diagonal k-step on: 64 consecutive
+    // doubles each
+    const double *K0 = K + (((size_t)p * NR + R0) * (NPP >> 2) + 4 * R0) * 64 + lane;
+    const double *K1 = K + (((size_t)p * NR + R1) * (NPP >> 2) + 4 * R1) * 64 + lane;
+    double av0[32], av1[16];
 #pragma unroll
-    for (int h = 0; h < 2; h++) {
-      const int R = h == 0 ? Rp : NR - 1 - Rp;
-      if (h == 1 && R == Rp) break;
-      rtile[q][h] = R;
-      const int nk = (NPP - 16 * R) >> 2;  // k-steps from the diagonal tile on
-      // A fragments of row tile R from its diagonal k-step on: 64 consecutive
-      // doubles each
-      const double *Kr = K + (((size_t)p * NR + R) * (NPP >> 2) + 4 * R) * 64 + lane;
-      double av[32];
+    for (int k = 0; k < 32; k++) av0[k] = k < nk0 ? K0[64 * k] : 0.0;
 #pragma unroll
-      for (int k = 0; k < 32; k++) av[k] = k < nk ? Kr[64 * k] : 0.0;
-      d4 D0 = {0.0, 0.0, 0.0, 0.0}, D1 = {0.0, 0.0, 0.0, 0.0};
+    for (int k = 0; k < 16; k++) av1[k] = k < nk1 ? K1[64 * k] : 0.0;
+    d4 D00 = {0.0, 0.0, 0.0, 0.0}, D01 = D00, D10 = D00, D11 = D00;
 #pragma unroll
-      for (int k = 0; k < 32; k++) {
-        if (k < nk) {
-          const double *b = Bp + (16 * R + 4 * k) * LTP;
-          D0 = __builtin_amdgcn_mfma_f64_16x16x4f64(av[k], b[0], D0, 0, 0, 0);
-          D1 = __builtin_amdgcn_mfma_f64_16x16x4f64(av[k], b[16], D1, 0, 0, 0);
-        }
+    for (int k = 0; k < 32; k++) {
+      if (k < nk0) {
+        const double *b = Bp + (16 * R0 + 4 * k) * LTP;
+        D00 = __builtin_amdgcn_mfma_f64_16x16x4f64(av0[k], b[0], D00, 0, 0, 0);
+        D01 = __builtin_amdgcn_mfma_f64_16x16x4f64(av0[k], b[16], D01, 0, 0, 0);
       }
-      res[q][h][0] = D0;
-      res[q][h][1] = D1;
+      if (k < 16 && k < nk1) {
+        const double *b = Bp + (16 * R1 + 4 * k) * LTP;
+        D10 = __builtin_amdgcn_mfma_f64_16x16x4f64(av1[k], b[0], D10, 0, 0, 0);
+        D11 = __builtin_amdgcn_mfma_f64_16x16x4f64(av1[k], b[16], D11, 0, 0, 0);
+      }
     }
+    res[q][0][0] = D00;
+    res[q][0][1] = D01;
+    res[q][1][0] = D10;
+    res[q][1][1] = D11;
   }
   __syncthreads();  // every wave is done with the input tile
 #pragma unroll
