@@ -182,3 +182,25 @@ def test_split_form_nodal(amd, orc):
     g1 = p.execute(u, 0)
     assert relerr(g1, g0) <= 1e-15
     assert relerr(g1, orc.FMMPlan(N, 64, 18, 0, 1).execute(u, 0)) <= tol(N)
+
+
+@pytest.mark.parametrize("N", [1 << 22, 3000001])
+def test_large(amd, orc, N):
+    """The largest sizes: 2^22 (L = 14, s = 64) and an odd length whose leaf
+    size (46) takes the general downward kernel; both directions against the
+    oracle, L2C rows 0-63 and sampled rows against the double-double sum."""
+    announce(f"large N={N}")
+    p = amd.FMMPlan(N, 64, 18, amd.BOTH)
+    op = orc.FMMPlan(N, 64, 18, orc.BOTH)
+    for d in (0, 1):
+        u = amd.coefficients(N, seed=900 + d, decay=1.0 if d else 0.0)
+        g = p.execute(u, d)
+        o = op.execute(u, d)
+        e = relerr(g, o)
+        print(f"  dir {d}: rel err vs oracle {e:.3e}")
+        assert e <= tol(N)
+        if d == 0:
+            rows = np.unique(np.concatenate([np.arange(64),
+                                             np.random.default_rng(N).integers(0, N, 64)]))
+            ref = orc.direct_dd_rows(u, rows, 0)
+            assert float(np.max(np.abs(g[rows] - ref))) / float(np.max(np.abs(o))) <= tol(N)
