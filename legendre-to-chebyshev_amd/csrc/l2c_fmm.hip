@@ -1687,11 +1687,13 @@ static int launch_direct_lines(const direct_plan_s *dp, const double *d_in, doub
 // ---------------------------------------------------------------------------
 // host orchestration
 
-// Workspace for a batch of lines: grown on demand, owned by the plan.
+// Workspace for a batch of lines: grown on demand, owned by the plan.  Calls
+// may have run on a caller's stream as well as the plan's own, so the device
+// is drained before the old workspace is released.
 static int ensure_ws(fmm_plan_s *p, size_t nlines, double **w) {
   const size_t need = std::max<size_t>(nlines, 1) * ws_doubles(p->geo);
   if (p->wl_cap < need) {
-    L2C_HIP(hipStreamSynchronize(p->stream), -1);
+    L2C_HIP(hipDeviceSynchronize(), -1);
     if (p->d_wl) (void)hipFree(p->d_wl);
     p->d_wl = nullptr;
     p->wl_cap = 0;
