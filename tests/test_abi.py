@@ -121,6 +121,28 @@ def test_invalid_arguments_fail_cleanly(amd):
     assert lib.l2c_last_error()
 
 
+def test_batch_and_precision_arguments_fail_cleanly(amd):
+    """The extension entry points check their arguments before touching the
+    device: NULL or -1 / 0 with a last-error text."""
+    import ctypes as C
+    lib = amd.lib()
+    n = (C.c_size_t * 3)(100, 0, 5000)
+    for args in ((None, 3, 18, 0, 1),   # no lengths
+                 (n, 0, 18, 0, 1),      # empty batch
+                 (n, 3, 18, 0, 1),      # a zero length
+                 ((C.c_size_t * 1)(100), 1, 1, 0, 1),    # rank below the supported range
+                 ((C.c_size_t * 1)(100), 1, 18, 3, 1)):  # unknown direction
+        assert not lib.l2c_create_batch(*args)
+        assert lib.l2c_last_error()
+    assert lib.l2c_execute_batch_device(None, None, None, 0, None) == -1
+    assert lib.l2c_execute_batch(None, None, None, 0) == -1
+    assert lib.l2c_batch_num_classes(None) == 0
+    assert lib.l2c_batch_get_class(None, 0, None) == -1
+    assert lib.l2c_batch_graph(None, 1) == -1
+    assert lib.l2c_set_precision(None, 1) == -1
+    assert lib.l2c_last_error()
+
+
 def test_plan_wrapper_reports_errors(amd):
     with pytest.raises(amd.L2CError):
         amd.FMMPlan(1000, 64, 1, 0)
