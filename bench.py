@@ -341,6 +341,9 @@ def main():
                     help="also time this maxs (the published figure's leaf size); 0 = skip")
     ap.add_argument("--pcie-steps", type=int, default=20,
                     help="host-pointer execute() calls timed for the PCIe-inclusive rate; 0 = skip")
+    ap.add_argument("--lines", type=int, default=8,
+                    help="also time this many vectors of length N per step in one batched call; "
+                         "0 or 1 = skip")
     ap.add_argument("--split-steps", type=int, default=200,
                     help="steps timed with precision form 1 (split M2L matrices); 0 = skip")
     ap.add_argument("--batch", type=int, default=4096,
@@ -455,6 +458,31 @@ def main():
                 "traffic": pmc_traffic(workload, dom), "algorithmic_bytes": alg,
                 "avg_launch_us": round(avg_us, 3)}
 
+    # the same transform on B independent vectors per step, one batched
+    # launch sequence (l2c_execute_lines_device): what the GPU sustains when
+    # transforms of one size arrive together (reported beside the value)
+    lines = None
+    if args.lines > 1:
+        B = args.lines
+        ub = np.concatenate([amd.coefficients(N, seed=1 + b, decay=args.decay) for b in range(B)])
+        bi, bo = amd.DeviceBuffer(B * N), amd.DeviceBuffer(B * N)
+        bi.upload(ub)
+        for _ in range(max(2, args.warmup // 8)):
+            plan.execute_lines_device(bi.ptr, bo.ptr, d, B, B, N, 0, 1)
+        plan.synchronize()
+        ls = max(5, args.steps // 8)
+        t0 = time.perf_counter()
+        for _ in range(ls):
+            plan.execute_lines_device(bi.ptr, bo.ptr, d, B, B, N, 0, 1)
+        plan.synchronize()
+        el_l = time.perf_counter() - t0
+        gb = bo.download()
+        e_l = float(np.max(np.abs(gb[:N] - o)) / np.max(np.abs(o)))
+        lines = {"vectors_per_step": B, "value": round(B * ls / el_l, 2), "unit": "transforms/s",
+                 "ms_per_step": round(1e3 * el_l / ls, 4), "max_rel_err_vs_oracle_first": e_l}
+        log(f"[bench] {B} vectors per step: {lines}")
+        del bi, bo
+
     # precision form 1 (split M2L matrices) on the same plan and input,
     # reported beside the FP64 value
     split = None
@@ -554,6 +582,7 @@ def main():
         "cpu_baseline": cpu,
         "host_pointer_transforms_s": pcie,
         "split_form": split,
+        "lines_per_step": lines,
         "batched": batched,
         "at_published_leaf_size": cmp_line,
     }
@@ -562,6 +591,8 @@ def main():
     if cmp_line is not None and not cmp_line["max_rel_err_vs_oracle"] <= tol:
         bad = True
     if split is not None and split.get("failed"):
+        bad = True
+    if lines is not None and not lines["max_rel_err_vs_oracle_first"] <= tol:
         bad = True
     if batched is not None and not batched["max_rel_err_vs_oracle"] <= 1e-14 * math.log2(8192):
         bad = True
