@@ -192,9 +192,34 @@ typedef struct {
 size_t l2c_batch_num_classes(const l2c_batch_plan *bp);
 int l2c_batch_get_class(const l2c_batch_plan *bp, size_t i, l2c_batch_class *out);
 
-/* Precision form of a plan.  0 = plain FP64 (default and, in this build,
- * the only form: any other value returns -1 with an error text). */
+/* Precision form of a plan.  0 = plain FP64 (default).  1 = split form: the
+ * M2L matrices are stored per coefficient as FP64, FP32, BF16 or dropped,
+ * chosen against 2^-29, 2^-44 and 2^-53 of each block's largest coefficient;
+ * all arithmetic stays FP64.  Any other value returns -1 with an error text. */
 int l2c_set_precision(fmm_plan *plan, int form);
+
+/* Length-N DCT-II and DCT-III of real FP64 vectors on the GPU: the steps
+ * between function values at the N Chebyshev-Gauss points and Chebyshev
+ * coefficients around L2C / C2L.  No counterpart in the reference library;
+ * it replaces the FFTW REDFT10 the reference times as its yardstick
+ * (src/bin/l2c.c:414-463, fftw_plan_r2r_1d(N, ..., FFTW_REDFT10, ...)).
+ * Unnormalised, as FFTW:
+ *   kind 2 (REDFT10): y_k = 2 sum_{n<N} x_n cos(pi k (2n+1) / (2N))
+ *   kind 3 (REDFT01): y_k = x_0 + 2 sum_{1<=n<N} x_n cos(pi n (2k+1) / (2N))
+ * so REDFT01(REDFT10(x)) = 2N x.  l2c_create_dct plans `count` vectors of
+ * length N stored back to back (N a power of two, 1 <= N <= 2^25) and returns
+ * NULL on error.  In-place calls (in == out) are allowed. */
+typedef struct l2c_dct_plan_s l2c_dct_plan;
+l2c_dct_plan *l2c_create_dct(size_t N, size_t count);
+void l2c_free_dct(l2c_dct_plan *p);
+/* d_in, d_out: device arrays of count * N doubles, 16-byte aligned.  Runs on
+ * `stream` (a hipStream_t, NULL = the default stream) without synchronising
+ * the host; calls that share a plan must be ordered on one stream, since they
+ * share its workspace.  Returns 0 or -1. */
+int l2c_execute_dct_device(l2c_dct_plan *p, const double *d_in, double *d_out, int kind,
+                           void *stream);
+/* Same with host arrays (staged through pinned buffers; synchronous). */
+int l2c_execute_dct(l2c_dct_plan *p, const double *in, double *out, int kind);
 
 /* Per-kernel timing with HIP events recorded on the launch stream around
  * the execute kernels.  l2c_profile(plan, n) resets the sums and, for n >= 1,

@@ -127,6 +127,13 @@ def lib():
         L.l2c_batch_num_classes.argtypes = [vp]
         L.l2c_batch_get_class.restype = C.c_int
         L.l2c_batch_get_class.argtypes = [vp, _sz, C.POINTER(BatchClass)]
+        L.l2c_create_dct.restype = vp
+        L.l2c_create_dct.argtypes = [_sz, _sz]
+        L.l2c_free_dct.argtypes = [vp]
+        L.l2c_execute_dct_device.restype = C.c_int
+        L.l2c_execute_dct_device.argtypes = [vp, vp, vp, C.c_int, vp]
+        L.l2c_execute_dct.restype = C.c_int
+        L.l2c_execute_dct.argtypes = [vp, vp, vp, C.c_int]
         _lib = L
     return _lib
 
@@ -398,6 +405,56 @@ class BatchPlan:
         if getattr(self, "_plan", None):
             lib().l2c_free_batch(self._plan)
             self._plan = None
+
+
+class DCTPlan:
+    """Length-N DCT-II (kind 2, FFTW REDFT10) and DCT-III (kind 3, FFTW REDFT01)
+    of `count` real vectors stored back to back (l2c_create_dct).  Unnormalised
+    as FFTW: dct(dct(x, 2), 3) = 2 N x."""
+
+    def __init__(self, N, count=1):
+        self.N, self.count = int(N), int(count)
+        self._plan = lib().l2c_create_dct(self.N, self.count)
+        if not self._plan:
+            raise L2CError("l2c_create_dct: " + last_error())
+
+    def execute(self, u, kind=2, out=None):
+        u = np.ascontiguousarray(u, dtype=np.float64)
+        if u.size != self.N * self.count:
+            raise ValueError(f"input must hold count * N = {self.N * self.count} values")
+        if out is None:
+            out = np.empty_like(u)
+        if lib().l2c_execute_dct(self._plan, _ptr(u), _ptr(out), kind) != 0:
+            raise L2CError("l2c_execute_dct: " + last_error())
+        return out
+
+    def execute_device(self, d_in, d_out, kind=2, stream=None):
+        if lib().l2c_execute_dct_device(self._plan, d_in, d_out, kind, stream) != 0:
+            raise L2CError("l2c_execute_dct_device: " + last_error())
+
+    def __del__(self):
+        if getattr(self, "_plan", None):
+            lib().l2c_free_dct(self._plan)
+            self._plan = None
+
+
+def cheb_values(c, plan=None):
+    """Values at the Chebyshev-Gauss points x_j = cos(pi (2j+1) / (2N)) of the
+    Chebyshev series with coefficients c: one REDFT01 of (c_0, c_1/2, ...)."""
+    c = np.array(c, dtype=np.float64)
+    c[1:] *= 0.5
+    plan = plan or DCTPlan(c.size)
+    return plan.execute(c, 3)
+
+
+def cheb_coefficients(f, plan=None):
+    """Chebyshev coefficients of the interpolant of values f at the N
+    Chebyshev-Gauss points: REDFT10(f) / N with c_0 halved."""
+    f = np.ascontiguousarray(f, dtype=np.float64)
+    plan = plan or DCTPlan(f.size)
+    c = plan.execute(f, 2) / f.size
+    c[0] *= 0.5
+    return c
 
 
 class DeviceBuffer:

@@ -54,6 +54,10 @@ def lib():
         L.orc_h.argtypes = [_sz, _sz]
         L.orc_binom_B0.argtypes = [_sz, _dp]
         L.orc_dct2.argtypes = [_sz, _dp, _dp]
+        L.orc_redft_direct.restype = C.c_int
+        L.orc_redft_direct.argtypes = [_sz, C.c_int, _dp, _dp]
+        L.orc_redft_fft.restype = C.c_int
+        L.orc_redft_fft.argtypes = [_sz, C.c_int, _dp, _dp]
         L.orc_create_direct.restype = C.POINTER(_Direct)
         L.orc_create_direct.argtypes = [_sz, _sz, _sz]
         L.orc_free_direct.argtypes = [C.POINTER(_Direct)]
@@ -116,6 +120,20 @@ def dct2(F):
     out = np.empty_like(F)
     lib().orc_dct2(F.shape[0], _p(F), _p(out))
     return out
+
+
+def redft(x, kind, method="fft"):
+    """FFTW REDFT10 (kind 2) / REDFT01 (kind 3) of each row of x, in long
+    double: "direct" is the O(N^2) definition, "fft" a length-4N complex FFT
+    of the symmetric extension (N a power of two)."""
+    x = np.ascontiguousarray(x, dtype=np.float64)
+    rows = x.reshape(-1, x.shape[-1])
+    out = np.empty_like(rows)
+    fn = lib().orc_redft_direct if method == "direct" else lib().orc_redft_fft
+    for r in range(rows.shape[0]):
+        if fn(rows.shape[1], kind, _p(rows[r]), _p(out[r])) != 0:
+            raise ValueError("orc_redft: bad N or kind")
+    return out.reshape(x.shape)
 
 
 class FMMPlan:

@@ -141,6 +141,11 @@ def test_batch_and_precision_arguments_fail_cleanly(amd):
     assert lib.l2c_batch_graph(None, 1) == -1
     assert lib.l2c_set_precision(None, 1) == -1
     assert lib.l2c_last_error()
+    for N, count in ((0, 1), (12, 1), (64, 0), (1 << 26, 1)):
+        assert not lib.l2c_create_dct(N, count)
+        assert lib.l2c_last_error()
+    assert lib.l2c_execute_dct_device(None, None, None, 2, None) == -1
+    assert lib.l2c_execute_dct(None, None, None, 2) == -1
 
 
 def test_plan_wrapper_reports_errors(amd):

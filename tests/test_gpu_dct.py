@@ -1,0 +1,133 @@
+"""GPU parity of the length-N DCT-II / DCT-III (l2c_create_dct,
+l2c_execute_dct_device): the FFTW REDFT10 / REDFT01 transforms the reference
+times as its yardstick (src/bin/l2c.c:414-463), and the steps between values
+at the Chebyshev-Gauss points and Chebyshev coefficients around L2C / C2L.
+
+Every result is checked against the oracle's long-double DCT (a length-4N
+FFT of the symmetric extension, pinned to the O(N^2) definition on the CPU);
+the 2^25 case, too large for it, against scipy's pocketfft.
+Tolerance: relative max-norm error <= 1e-15 * log2(2N) per vector."""
+import math
+
+import numpy as np
+import pytest
+
+from conftest import announce, relerr
+
+pytestmark = pytest.mark.gpu
+
+
+def tol(N):
+    return 1e-15 * math.log2(2 * N)
+
+
+def run(amd, x, N, count, kind, inplace=False):
+    plan = amd.DCTPlan(N, count)
+    din = amd.DeviceBuffer(N * count)
+    dout = din if inplace else amd.DeviceBuffer(N * count)
+    din.upload(x.ravel())
+    plan.execute_device(din.ptr, dout.ptr, kind)
+    return dout.download().reshape(count, N)
+
+
+def check(amd, orc, N, count, kind, seed, inplace=False):
+    announce(f"dct kind={kind} N={N} count={count} inplace={inplace}")
+    x = np.random.default_rng(seed).uniform(-1, 1, (count, N))
+    y = run(amd, x, N, count, kind, inplace)
+    ref = orc.redft(x, kind)
+    errs = [relerr(y[r], ref[r]) for r in range(count)]
+    assert max(errs) < tol(N), (max(errs), tol(N))
+    return max(errs)
+
+
+@pytest.mark.parametrize("kind", [2, 3])
+@pytest.mark.parametrize("logN", list(range(0, 14)))
+def test_one_pass_sizes(amd, orc, kind, logN):
+    """N = 1 ... 4096: whole vectors in one work-group's LDS (several per
+    work-group below N = 4096), with a count that leaves the last
+    work-group partly empty; N = 8192 is the first two-pass size."""
+    N = 1 << logN
+    check(amd, orc, N, 1, kind, 10 * logN + kind)
+    check(amd, orc, N, 37, kind, 10 * logN + kind + 1)
+
+
+@pytest.mark.parametrize("kind", [2, 3])
+@pytest.mark.parametrize("logN", [14, 15, 16, 17, 18, 19, 20, 21])
+def test_multi_pass_sizes(amd, orc, kind, logN):
+    """N = 2^14 ... 2^16: two HBM passes (lengths 64 ... 256); N = 2^17 ...
+    2^21: three passes (lengths 32 ... 128)."""
+    N = 1 << logN
+    check(amd, orc, N, 1, kind, 100 + logN + kind)
+    if logN <= 17:
+        check(amd, orc, N, 3, kind, 200 + logN + kind)
+
+
+@pytest.mark.parametrize("kind", [2, 3])
+@pytest.mark.parametrize("logN,count", [(13, 1024), (16, 64), (17, 32), (20, 4)])
+def test_wide_work_groups(amd, orc, kind, logN, count):
+    """Jobs of at least 2^21 complex points run the passes with 4096 elements
+    per work-group (two passes at N = 2^13, three from N = 2^17 on, the middle
+    one wide from N = 2^18 on)."""
+    check(amd, orc, 1 << logN, count, kind, 300 + logN + kind)
+
+
+@pytest.mark.parametrize("kind", [2, 3])
+def test_largest_pass_lengths(amd, kind):
+    """N = 2^23 (the reference's largest timed size) and 2^25 (three passes
+    of 256, the largest plan) against scipy's pocketfft."""
+    sf = pytest.importorskip("scipy.fft")
+    for logN in (23, 25):
+        N = 1 << logN
+        announce(f"dct kind={kind} N=2^{logN}")
+        x = np.random.default_rng(logN + kind).uniform(-1, 1, N)
+        y = run(amd, x, N, 1, kind)[0]
+        assert relerr(y, sf.dct(x, type=kind)) < tol(N)
+
+
+@pytest.mark.parametrize("N", [256, 1 << 16])
+def test_in_place(amd, orc, N):
+    for kind in (2, 3):
+        check(amd, orc, N, 2, kind, N + kind, inplace=True)
+
+
+@pytest.mark.parametrize("N", [64, 1 << 15])
+def test_round_trip_and_host_entry(amd, N):
+    """REDFT01(REDFT10(x)) = 2N x through the host-array entry point, which
+    also equals the device entry point bit for bit."""
+    announce(f"dct round trip N={N}")
+    x = np.random.default_rng(N).uniform(-1, 1, 2 * N)
+    plan = amd.DCTPlan(N, 2)
+    y = plan.execute(x, 2)
+    assert np.array_equal(y.reshape(2, N), run(amd, x.reshape(2, N), N, 2, 2))
+    assert relerr(plan.execute(y, 3) / (2 * N), x) < tol(N)
+
+
+def test_values_pipeline(amd):
+    """Legendre coefficients -> L2C (FMM) -> Chebyshev coefficients -> DCT-III
+    -> values at the Chebyshev-Gauss points equal the Legendre series summed
+    directly there; values -> DCT-II -> C2L gives the coefficients back."""
+    from numpy.polynomial import legendre
+    N = 1 << 12
+    announce(f"dct values pipeline N={N}")
+    rng = np.random.default_rng(5)
+    a = rng.uniform(-1, 1, N) / (1 + np.arange(N)) ** 2
+    plan = amd.FMMPlan(N, 64, 18, amd.BOTH)
+    dct = amd.DCTPlan(N)
+    c = plan.execute(a, amd.L2C)
+    f = amd.cheb_values(c, dct)
+    xj = np.cos(np.pi * (2 * np.arange(N) + 1) / (2 * N))
+    assert relerr(f, legendre.legval(xj, a)) < 1e-13
+    back = plan.execute(amd.cheb_coefficients(f, dct), amd.C2L)
+    assert relerr(back, a) < 1e-13
+
+
+def test_errors(amd):
+    lib = amd.lib()
+    assert not lib.l2c_create_dct(12, 1)
+    assert "power of two" in amd.last_error()
+    plan = amd.DCTPlan(64)
+    buf = amd.DeviceBuffer(64)
+    assert lib.l2c_execute_dct_device(plan._plan, buf.ptr, buf.ptr, 4, None) == -1
+    assert "kind" in amd.last_error()
+    assert lib.l2c_execute_dct_device(plan._plan, buf.ptr + 8, buf.ptr, 2, None) == -1
+    assert "aligned" in amd.last_error()
