@@ -160,7 +160,8 @@ class Leg2Cheb:
                  precompute=0, verbose=1):
         self.N = int(input_array.shape[0])
         self.direction = direction
-        self._plan = lib().create_fmm(self.N, maxs, M, direction, lagrange, precompute, verbose)
+        self._lib = lib()
+        self._plan = self._lib.create_fmm(self.N, maxs, M, direction, lagrange, precompute, verbose)
         if not self._plan:
             raise L2CError("create_fmm: " + last_error())
         self._input_array = input_array
@@ -201,7 +202,7 @@ class Leg2Cheb:
 
     def __del__(self):
         if getattr(self, "_plan", None):
-            lib().free_fmm(self._plan)
+            self._lib.free_fmm(self._plan)
             self._plan = None
 
 
@@ -210,7 +211,8 @@ class FMMPlan:
 
     def __init__(self, N, maxs=64, M=18, direction=BOTH, lagrange=0, precompute=0, verbose=0):
         self.N = N
-        self._plan = lib().create_fmm(N, maxs, M, direction, lagrange, precompute, verbose)
+        self._lib = lib()
+        self._plan = self._lib.create_fmm(N, maxs, M, direction, lagrange, precompute, verbose)
         if not self._plan:
             raise L2CError("create_fmm: " + last_error())
 
@@ -281,7 +283,7 @@ class FMMPlan:
 
     def __del__(self):
         if getattr(self, "_plan", None):
-            lib().free_fmm(self._plan)
+            self._lib.free_fmm(self._plan)
             self._plan = None
 
 
@@ -289,7 +291,8 @@ class FMMPlan2D:
     """create_fmm_2d / execute2D / free_fmm_2d (src/C/leg2cheb.c:1236-1297)."""
 
     def __init__(self, N0, N1, axis, maxs=64, M=18, direction=BOTH, lagrange=0, precompute=0):
-        self._plan = lib().create_fmm_2d(N0, N1, axis, maxs, M, direction, lagrange, precompute, 0)
+        self._lib = lib()
+        self._plan = self._lib.create_fmm_2d(N0, N1, axis, maxs, M, direction, lagrange, precompute, 0)
         if not self._plan:
             raise L2CError("create_fmm_2d: " + last_error())
 
@@ -304,7 +307,7 @@ class FMMPlan2D:
 
     def __del__(self):
         if getattr(self, "_plan", None):
-            lib().free_fmm_2d(self._plan)
+            self._lib.free_fmm_2d(self._plan)
             self._plan = None
 
 
@@ -359,8 +362,9 @@ class BatchPlan:
         self.lengths = n.astype(np.int64)
         self.total = int(n.sum())
         self.offsets = np.concatenate([[0], np.cumsum(self.lengths)[:-1]]).astype(np.int64)
-        self._plan = lib().l2c_create_batch(n.ctypes.data_as(C.POINTER(_sz)), n.size, M,
-                                            direction, 1 if tune else 0)
+        self._lib = lib()
+        self._plan = self._lib.l2c_create_batch(n.ctypes.data_as(C.POINTER(_sz)), n.size, M,
+                                                direction, 1 if tune else 0)
         if not self._plan:
             raise L2CError("l2c_create_batch: " + last_error())
 
@@ -403,7 +407,7 @@ class BatchPlan:
 
     def __del__(self):
         if getattr(self, "_plan", None):
-            lib().l2c_free_batch(self._plan)
+            self._lib.l2c_free_batch(self._plan)
             self._plan = None
 
 
@@ -414,7 +418,8 @@ class DCTPlan:
 
     def __init__(self, N, count=1):
         self.N, self.count = int(N), int(count)
-        self._plan = lib().l2c_create_dct(self.N, self.count)
+        self._lib = lib()
+        self._plan = self._lib.l2c_create_dct(self.N, self.count)
         if not self._plan:
             raise L2CError("l2c_create_dct: " + last_error())
 
@@ -434,7 +439,7 @@ class DCTPlan:
 
     def __del__(self):
         if getattr(self, "_plan", None):
-            lib().l2c_free_dct(self._plan)
+            self._lib.l2c_free_dct(self._plan)
             self._plan = None
 
 
@@ -462,7 +467,8 @@ class DeviceBuffer:
 
     def __init__(self, n):
         self.n = n
-        self.ptr = lib().l2c_malloc_device(n * 8)
+        self._lib = lib()
+        self.ptr = self._lib.l2c_malloc_device(n * 8)
         if not self.ptr:
             raise L2CError("l2c_malloc_device: " + last_error())
 
@@ -481,7 +487,7 @@ class DeviceBuffer:
 
     def free(self):
         if self.ptr:
-            lib().l2c_free_device(self.ptr)
+            self._lib.l2c_free_device(self.ptr)
             self.ptr = None
 
     def __del__(self):
