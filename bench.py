@@ -11,6 +11,10 @@ against extended-precision direct sums on sampled rows.  The oracle, run on
 the host cores, is the CPU baseline.
 
     python bench.py [--steps K] [--warmup W] [--gpus 1]
+
+Other workloads, each one JSON line: --workload batch (4096 transforms of
+uneven length), --workload field (a 256^3 field along every axis),
+--workload dct (the length-N DCT-II, the reference's FFTW yardstick).
 """
 import argparse
 import importlib.util
@@ -313,13 +317,79 @@ def bench_field(amd, args):
         sys.exit(1)
 
 
+FFTW_REDFT10_2_20 = {"value": round(1 / 6.263459e-03, 2), "unit": "transforms/s",
+                     "source": "src/results/figure4.sh:137 (FFTW REDFT10, N=2^20, "
+                               "6.263459e-03 s, one core)"}
+
+
+def bench_dct(amd, orc, args, steps):
+    """Length-N DCT-II (FFTW REDFT10, the reference's yardstick,
+    src/bin/l2c.c:414-463 on fun[i] = i^2) of one vector resident in HBM,
+    and the same transform batched over many short vectors.  Algorithmic HBM
+    bytes: 16 per element (read and write once); the multi-pass form moves
+    16 more per element per pass boundary through its workspace."""
+    N = args.dct_n
+    x = np.arange(N, dtype=np.float64) ** 2
+    plan = amd.DCTPlan(N)
+    din, dout = amd.DeviceBuffer(N), amd.DeviceBuffer(N)
+    din.upload(x)
+    for _ in range(max(5, steps // 10)):
+        plan.execute_device(din.ptr, dout.ptr, 2)
+    plan.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        plan.execute_device(din.ptr, dout.ptr, 2)
+    plan.synchronize()
+    el = (time.perf_counter() - t0) / steps
+    ref = orc.redft(x, 2)
+    err = float(np.max(np.abs(dout.download() - ref)) / np.max(np.abs(ref)))
+    logm = int(math.log2(N)) - 1
+    passes = 1 if logm <= 11 else (2 if logm < 16 else 3)
+    out = {"N": N, "kind": "REDFT10", "us": round(1e6 * el, 3),
+           "value": round(1 / el, 1), "unit": "transforms/s",
+           "vs_fftw_published": round(1 / el / FFTW_REDFT10_2_20["value"], 1) if N == 1 << 20 else None,
+           "algorithmic_GBps": round(16 * N / el / 1e9, 1),
+           "hbm_passes": passes, "traffic_bytes": 16 * N * passes,
+           "max_rel_err_vs_oracle": err}
+    # many short vectors in one call: N = 1024, 8192 of them (64 MB each way)
+    n_b, c_b = 1024, 8192
+    pb = amd.DCTPlan(n_b, c_b)
+    xb = amd.splitmix_uniform(n_b * c_b, 11)
+    bi, bo = amd.DeviceBuffer(n_b * c_b), amd.DeviceBuffer(n_b * c_b)
+    bi.upload(xb)
+    for _ in range(5):
+        pb.execute_device(bi.ptr, bo.ptr, 2)
+    pb.synchronize()
+    sb = max(10, steps // 4)
+    t0 = time.perf_counter()
+    for _ in range(sb):
+        pb.execute_device(bi.ptr, bo.ptr, 2)
+    pb.synchronize()
+    eb = (time.perf_counter() - t0) / sb
+    yb = bo.download().reshape(c_b, n_b)
+    rows = [0, 1, c_b // 2, c_b - 1]
+    errb = max(float(np.max(np.abs(yb[r] - orc.redft(xb.reshape(c_b, n_b)[r], 2)))
+                     / np.max(np.abs(orc.redft(xb.reshape(c_b, n_b)[r], 2)))) for r in rows)
+    out["batched"] = {"N": n_b, "count": c_b, "us": round(1e6 * eb, 2),
+                      "transforms_s": round(c_b / eb, 1),
+                      "algorithmic_GBps": round(16 * n_b * c_b / eb / 1e9, 1),
+                      "max_rel_err_vs_oracle_rows": errb}
+    out["tol"] = 1e-15 * math.log2(2 * N)
+    log(f"[bench] dct: {out}")
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser(description=__doc__, formatter_class=argparse.RawDescriptionHelpFormatter)
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--workload", choices=("fmm", "batch", "field"), default="fmm",
+    ap.add_argument("--workload", choices=("fmm", "batch", "field", "dct"), default="fmm",
                     help="fmm: the N=2^20 transform and the uneven batch (default); batch: the "
                          "uneven batch alone; field: the 256^3 strided field with 2^18 vectors "
                          "mixed in")
+    ap.add_argument("--dct-n", type=int, default=1 << 20,
+                    help="length of the DCT-II of the dct workload and of the default line's dct entry")
+    ap.add_argument("--dct-steps", type=int, default=200,
+                    help="timed DCT-II calls in the default line (0 = skip)")
     ap.add_argument("--field-n", type=int, default=256)
     ap.add_argument("--field-vectors", type=int, default=64)
     ap.add_argument("--steps", type=int, default=400)
@@ -377,6 +447,51 @@ def main():
     amd = load_pkg()
     if args.workload == "field":
         bench_field(amd, args)
+        return
+    if args.workload == "dct":
+        sys.path.insert(0, ROOT)
+        from oracle import orc
+        if args.copy_mb:
+            log(f"[bench] calibration copy {args.copy_mb} MiB")
+            copy_calibration(amd, args.copy_mb)
+        dc = bench_dct(amd, orc, args, args.steps)
+        cpu = None
+        if not args.no_cpu:
+            x = np.arange(args.dct_n, dtype=np.float64) ** 2
+            reps, t0 = 0, time.perf_counter()
+            while time.perf_counter() - t0 < args.cpu_seconds:
+                orc.redft(x, 2)
+                reps += 1
+            el_c = time.perf_counter() - t0
+            cpu = {"value": round(reps / el_c, 3), "unit": "transforms/s", "cores": 1, "kind": "port",
+                   "sample": f"oracle REDFT10 (long-double length-4N FFT), N={args.dct_n}, "
+                             f"{reps} repeats in {el_c:.1f} s"}
+            try:
+                import scipy.fft as sfft
+                reps, t0 = 0, time.perf_counter()
+                while time.perf_counter() - t0 < args.cpu_seconds / 2:
+                    sfft.dct(x, type=2, workers=1)
+                    reps += 1
+                cpu["scipy_pocketfft_one_core"] = round(reps / (time.perf_counter() - t0), 1)
+            except ImportError:
+                pass
+        line = {"metric": f"DCT-II (FFTW REDFT10) transforms/s at N={args.dct_n}, fun[i] = i^2",
+                "value": dc["value"], "unit": "transforms/s", "n_gpus": 1, "steps": args.steps,
+                "warmup": max(5, args.steps // 10), "ms_per_step": round(dc["us"] / 1e3, 5),
+                "higher_is_better": True, "scaling": "weak",
+                "vs_baseline": dc["vs_fftw_published"], "baseline": FFTW_REDFT10_2_20,
+                "dtype": "f64", "data": "fun[i] = i^2 as in the reference's test_dct0",
+                "config": {"workload": f"dct2_N{args.dct_n}", "N": args.dct_n, "kind": "REDFT10",
+                           "inputs_resident": "HBM"},
+                "roofline": {"bound": "hbm", "achieved": dc["algorithmic_GBps"], "peak": 8000.0,
+                             "unit": "GB/s", "frac": round(dc["algorithmic_GBps"] / 8000.0, 4),
+                             "traffic": dc["traffic_bytes"]},
+                "dct": dc, "cpu_baseline": cpu}
+        print(json.dumps(line), flush=True)
+        if not (dc["max_rel_err_vs_oracle"] <= dc["tol"]
+                and dc["batched"]["max_rel_err_vs_oracle_rows"] <= 1e-15 * math.log2(2048)):
+            log("[bench] FAILED: DCT error above tolerance")
+            sys.exit(1)
         return
     if args.workload == "batch":
         sys.path.insert(0, ROOT)
@@ -546,6 +661,7 @@ def main():
         del p2
 
     batched = bench_batch(amd, orc, args) if args.batch > 0 else None
+    dct = bench_dct(amd, orc, args, args.dct_steps) if args.dct_steps > 0 else None
 
     cpu = None
     if not args.no_cpu:
@@ -598,6 +714,7 @@ def main():
         "split_form": split,
         "lines_per_step": lines,
         "batched": batched,
+        "dct2": dct,
         "at_published_leaf_size": cmp_line,
     }
     print(json.dumps(line), flush=True)
@@ -609,6 +726,9 @@ def main():
     if lines is not None and not lines["max_rel_err_vs_oracle_first"] <= tol:
         bad = True
     if batched is not None and not batched["max_rel_err_vs_oracle"] <= 1e-14 * math.log2(8192):
+        bad = True
+    if dct is not None and not (dct["max_rel_err_vs_oracle"] <= dct["tol"]
+                                and dct["batched"]["max_rel_err_vs_oracle_rows"] <= 1e-15 * math.log2(2048)):
         bad = True
     if bad:
         log(f"[bench] FAILED: error above {tol:.2e}")

@@ -220,6 +220,8 @@ int l2c_execute_dct_device(l2c_dct_plan *p, const double *d_in, double *d_out, i
                            void *stream);
 /* Same with host arrays (staged through pinned buffers; synchronous). */
 int l2c_execute_dct(l2c_dct_plan *p, const double *in, double *out, int kind);
+/* Wait for all work on the plan's device. */
+int l2c_dct_synchronize(l2c_dct_plan *p);
 
 /* Per-kernel timing with HIP events recorded on the launch stream around
  * the execute kernels.  l2c_profile(plan, n) resets the sums and, for n >= 1,

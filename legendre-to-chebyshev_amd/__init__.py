@@ -134,6 +134,8 @@ def lib():
         L.l2c_execute_dct_device.argtypes = [vp, vp, vp, C.c_int, vp]
         L.l2c_execute_dct.restype = C.c_int
         L.l2c_execute_dct.argtypes = [vp, vp, vp, C.c_int]
+        L.l2c_dct_synchronize.restype = C.c_int
+        L.l2c_dct_synchronize.argtypes = [vp]
         _lib = L
     return _lib
 
@@ -436,6 +438,10 @@ class DCTPlan:
     def execute_device(self, d_in, d_out, kind=2, stream=None):
         if lib().l2c_execute_dct_device(self._plan, d_in, d_out, kind, stream) != 0:
             raise L2CError("l2c_execute_dct_device: " + last_error())
+
+    def synchronize(self):
+        if lib().l2c_dct_synchronize(self._plan) != 0:
+            raise L2CError(last_error())
 
     def __del__(self):
         if getattr(self, "_plan", None):

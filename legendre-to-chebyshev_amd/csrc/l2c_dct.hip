@@ -938,6 +938,21 @@ l2c_dct_plan *l2c_create_dct(size_t N, size_t count) { return create_dct_impl(N,
 
 void l2c_free_dct(l2c_dct_plan *p) { dct_release(p); }
 
+int l2c_dct_synchronize(l2c_dct_plan *p) {
+  clear_error();
+  if (!p) {
+    set_error("l2c_dct_synchronize: NULL plan");
+    return -1;
+  }
+  int cur = 0;
+  L2C_HIP(hipGetDevice(&cur), -1);
+  if (cur != p->dev) L2C_HIP(hipSetDevice(p->dev), -1);
+  const hipError_t e = hipDeviceSynchronize();
+  if (cur != p->dev) (void)hipSetDevice(cur);
+  L2C_HIP(e, -1);
+  return 0;
+}
+
 int l2c_execute_dct_device(l2c_dct_plan *p, const double *d_in, double *d_out, int kind,
                            void *stream) {
   clear_error();

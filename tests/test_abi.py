@@ -146,6 +146,7 @@ def test_batch_and_precision_arguments_fail_cleanly(amd):
         assert lib.l2c_last_error()
     assert lib.l2c_execute_dct_device(None, None, None, 2, None) == -1
     assert lib.l2c_execute_dct(None, None, None, 2) == -1
+    assert lib.l2c_dct_synchronize(None) == -1
 
 
 def test_plan_wrapper_reports_errors(amd):
