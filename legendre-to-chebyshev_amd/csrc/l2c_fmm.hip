@@ -1531,28 +1531,27 @@ __global__ void __launch_bounds__(256) k_direct(const double *__restrict__ in,
 // tile of LT lines, K_p upper triangular (the direct sums of
 // src/C/leg2cheb.c:152-234 with scaling and C2L weights folded in).
 //
-// LDS holds the tile's input as [p][c][line] (row stride kDenseLTP); MFMA
-// v_mfma_f64_16x16x4f64 with A = K_p from global memory (the plan's table,
-// L2-resident), B = the input rows from LDS.  Row tiles R and NR-1-R are
-// paired so that every wave runs the same number of k-steps; a wave keeps its
-// results in registers until all waves are done with the input, then the
-// tile is written back through LDS with the loads' coalescing.
-constexpr int kDenseLT = 32, kDenseLTP = 36, kDenseNT = 256;
-__global__ void __launch_bounds__(kDenseNT) k_lines_dense(const double *__restrict__ in,
-                                                          double *__restrict__ out, LineMap lm,
-                                                          size_t nlines, int N, int NPP,
-                                                          const double *__restrict__ K) {
+// The two parities run one after the other through one LDS tile [c][line]
+// (row stride kDenseLTP: 36 KB at NPP = 128) and the kernel fits 128 VGPRs,
+// so four work-groups share a CU and one's loads overlap another's products.
+// v_mfma_f64_16x16x4f64 with A =
+// K_p from global memory (the plan's table, L2-resident, in fragment order,
+// streamed kDenseKG k-steps ahead) and B = the input rows from LDS.  Wave w
+// owns row tiles w and NR-1-w, so every wave runs the same number of k-steps;
+// results go back through the LDS tile so that the stores keep the loads'
+// coalescing.
+constexpr int kDenseLT = 32, kDenseLTP = 36, kDenseNT = 256, kDenseKG = 4;
+__global__ void __launch_bounds__(kDenseNT) __attribute__((amdgpu_waves_per_eu(4)))
+    k_lines_dense(const double *__restrict__ in, double *__restrict__ out, LineMap lm,
+                  size_t nlines, int N, int NPP, const double *__restrict__ K) {
   extern __shared__ double sm[];
   const int tid = threadIdx.x, lane = tid & 63;
   const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
   const size_t l0 = (size_t)blockIdx.x * kDenseLT;
-  const int LTP = kDenseLTP, NE = 2 * NPP;  // NE <= 256: 32 elements per thread
+  const int LTP = kDenseLTP;
   const bool contig = lm.es == 1;
-  // the tile's elements (j, l): lanes run along j when the lines are
-  // contiguous, along l (32 lines, two rows j per wave) otherwise, so that
-  // neighbouring lanes touch neighbouring words
   // start and length of the tile's lines, computed once (line_off divides)
-  size_t *toff = (size_t *)(sm + 2 * NPP * LTP);
+  size_t *toff = (size_t *)(sm + NPP * LTP);
   int *tlen = (int *)(toff + kDenseLT);
   if (tid < kDenseLT) {
     const size_t line = l0 + tid;
@@ -1560,111 +1559,104 @@ __global__ void __launch_bounds__(kDenseNT) k_lines_dense(const double *__restri
     tlen[tid] = line < nlines ? line_len(lm, line, N) : 0;
   }
   __syncthreads();
-  auto in_at = [&](int j, int l) -> double {
-    return j < tlen[l] ? in[toff[l] + (size_t)j * lm.es] : 0.0;
+  // element q < 16 of this thread in one parity's tile (c < NPP <= 128, 32
+  // lines): lanes run along c when the lines are contiguous, along the lines
+  // (two rows c per wave) otherwise, so neighbouring lanes touch neighbouring
+  // words of one parity
+  auto tile_c = [&](int q) { return contig ? lane + 64 * (q & 1) : 2 * wave + (lane >> 5) + 8 * q; };
+  auto tile_l = [&](int q) { return contig ? wave + 4 * (q >> 1) : (lane & 31); };
+  auto load = [&](int p, double *vt) {
+#pragma unroll
+    for (int q = 0; q < 16; q++) {
+      const int c = tile_c(q), l = tile_l(q), j = 2 * c + p;
+      vt[q] = c < NPP && j < tlen[l] ? in[toff[l] + (size_t)j * lm.es] : 0.0;
+    }
   };
-  // element q of this thread (q < 32 covers NE <= 256, LT = 32):
-  //   contiguous lines: line wave + 4 (q / 4), j = lane + 64 (q % 4)
-  //   otherwise:        line lane % 32,        j = 2 wave + lane / 32 + 8 q
-  // all 32 loads are issued before the first LDS store
-  auto tile_j = [&](int q) { return contig ? lane + 64 * (q & 3) : 2 * wave + (lane >> 5) + 8 * q; };
-  auto tile_l = [&](int q) { return contig ? wave + 4 * (q >> 2) : (lane & 31); };
-  double vt[32];
-#pragma unroll
-  for (int q = 0; q < 32; q++) {
-    const int j = tile_j(q);
-    vt[q] = j < NE ? in_at(j, tile_l(q)) : 0.0;
-  }
-#pragma unroll
-  for (int q = 0; q < 32; q++) {
-    const int j = tile_j(q);
-    if (j < NE) sm[(j & 1) * NPP * LTP + (j >> 1) * LTP + tile_l(q)] = vt[q];
-  }
-  __syncthreads();
-
   const int NR = NPP >> 4, npair = (NR + 1) >> 1;
-  d4 res[2][2][2];  // [pair slot][row tile of the pair][column tile]
-  int rtile[2][2], rpar[2];
+  const bool active = wave < npair;
+  const int R0 = wave, R1 = NR - 1 - wave;
+  const bool two = active && R1 != R0;
+  const int nk0 = active ? (NPP - 16 * R0) >> 2 : 0, nk1 = two ? (NPP - 16 * R1) >> 2 : 0;
+  const double *Bp = sm + (lane >> 4) * LTP + (lane & 15);
+  double vt[16];
+  load(0, vt);
+  for (int p = 0; p < 2; p++) {
 #pragma unroll
-  for (int q = 0; q < 2; q++) {
-    rpar[q] = -1;
-    rtile[q][0] = rtile[q][1] = -1;
-#pragma unroll
-    for (int h = 0; h < 2; h++)
-#pragma unroll
-      for (int c = 0; c < 2; c++) res[q][h][c] = d4{0.0, 0.0, 0.0, 0.0};
-  }
-#pragma unroll
-  for (int q = 0; q < 2; q++) {
-    const int jp = wave + 4 * q;  // pair index: parity jp & 1, row tiles jp/2 and NR-1-jp/2
-    if (jp >= 2 * npair) break;
-    const int p = jp & 1, Rp = jp >> 1;
-    rpar[q] = p;
-    const double *Bp = sm + p * NPP * LTP + (lane >> 4) * LTP + (lane & 15);
-    // the pair's two row tiles run together: four independent MFMA chains
-    // (row tile x column tile); the second tile is the shorter one
-    const int R0 = Rp, R1 = NR - 1 - Rp;
-    const bool two = R1 != R0;
-    rtile[q][0] = R0;
-    rtile[q][1] = two ? R1 : -1;
-    const int nk0 = (NPP - 16 * R0) >> 2, nk1 = two ? (NPP - 16 * R1) >> 2 : 0;
-    // A fragments of a row tile from its diagonal k-step on: 64 consecutive
-    // doubles each
-    const double *K0 = K + (((size_t)p * NR + R0) * (NPP >> 2) + 4 * R0) * 64 + lane;
-    const double *K1 = K + (((size_t)p * NR + R1) * (NPP >> 2) + 4 * R1) * 64 + lane;
-    double av0[32], av1[16];
-#pragma unroll
-    for (int k = 0; k < 32; k++) av0[k] = k < nk0 ? K0[64 * k] : 0.0;
-#pragma unroll
-    for (int k = 0; k < 16; k++) av1[k] = k < nk1 ? K1[64 * k] : 0.0;
+    for (int q = 0; q < 16; q++) {
+      const int c = tile_c(q);
+      if (c < NPP) sm[c * LTP + tile_l(q)] = vt[q];
+    }
+    __syncthreads();
     d4 D00 = {0.0, 0.0, 0.0, 0.0}, D01 = D00, D10 = D00, D11 = D00;
+    if (active) {
+      // A fragments of row tile R from its diagonal k-step on: 64 doubles each
+      const double *K0 = K + (((size_t)p * NR + R0) * (NPP >> 2) + 4 * R0) * 64 + lane;
+      const double *K1 = K + (((size_t)p * NR + R1) * (NPP >> 2) + 4 * R1) * 64 + lane;
+      constexpr int G = kDenseKG, NG = 32 / G;  // nk0 <= 32, nk1 <= 16
+      double a0[2][G], a1[2][G];
 #pragma unroll
-    for (int k = 0; k < 32; k++) {
-      if (k < nk0) {
-        const double *b = Bp + (16 * R0 + 4 * k) * LTP;
-        D00 = __builtin_amdgcn_mfma_f64_16x16x4f64(av0[k], b[0], D00, 0, 0, 0);
-        D01 = __builtin_amdgcn_mfma_f64_16x16x4f64(av0[k], b[16], D01, 0, 0, 0);
+      for (int kk = 0; kk < G; kk++) {
+        a0[0][kk] = kk < nk0 ? K0[64 * kk] : 0.0;
+        a1[0][kk] = kk < nk1 ? K1[64 * kk] : 0.0;
       }
-      if (k < 16 && k < nk1) {
-        const double *b = Bp + (16 * R1 + 4 * k) * LTP;
-        D10 = __builtin_amdgcn_mfma_f64_16x16x4f64(av1[k], b[0], D10, 0, 0, 0);
-        D11 = __builtin_amdgcn_mfma_f64_16x16x4f64(av1[k], b[16], D11, 0, 0, 0);
+#pragma unroll
+      for (int gi = 0; gi < NG; gi++) {
+        const int cur = gi & 1;
+        if (gi + 1 < NG) {
+#pragma unroll
+          for (int kk = 0; kk < G; kk++) {
+            const int k = G * (gi + 1) + kk;
+            a0[cur ^ 1][kk] = k < nk0 ? K0[64 * k] : 0.0;
+            if (k < 16) a1[cur ^ 1][kk] = k < nk1 ? K1[64 * k] : 0.0;
+          }
+        }
+#pragma unroll
+        for (int kk = 0; kk < G; kk++) {
+          const int k = G * gi + kk;
+          if (k < nk0) {
+            const double *b = Bp + (16 * R0 + 4 * k) * LTP;
+            D00 = __builtin_amdgcn_mfma_f64_16x16x4f64(a0[cur][kk], b[0], D00, 0, 0, 0);
+            D01 = __builtin_amdgcn_mfma_f64_16x16x4f64(a0[cur][kk], b[16], D01, 0, 0, 0);
+          }
+          if (k < 16 && k < nk1) {
+            const double *b = Bp + (16 * R1 + 4 * k) * LTP;
+            D10 = __builtin_amdgcn_mfma_f64_16x16x4f64(a1[cur][kk], b[0], D10, 0, 0, 0);
+            D11 = __builtin_amdgcn_mfma_f64_16x16x4f64(a1[cur][kk], b[16], D11, 0, 0, 0);
+          }
+        }
       }
     }
-    res[q][0][0] = D00;
-    res[q][0][1] = D01;
-    res[q][1][0] = D10;
-    res[q][1][1] = D11;
-  }
-  __syncthreads();  // every wave is done with the input tile
+    __syncthreads();  // every wave is done with the input tile
+    if (active) {
+      double *o = sm + (16 * R0 + (lane >> 4)) * LTP + (lane & 15);
 #pragma unroll
-  for (int q = 0; q < 2; q++) {
-    if (rpar[q] < 0) break;
+      for (int v = 0; v < 4; v++) {
+        o[4 * v * LTP] = D00[v];
+        o[4 * v * LTP + 16] = D01[v];
+      }
+      if (two) {
+        o = sm + (16 * R1 + (lane >> 4)) * LTP + (lane & 15);
 #pragma unroll
-    for (int h = 0; h < 2; h++) {
-      const int R = rtile[q][h];
-      if (R < 0) break;
-      double *o = sm + rpar[q] * NPP * LTP + (16 * R + (lane >> 4)) * LTP + (lane & 15);
-#pragma unroll
-      for (int c = 0; c < 2; c++)
-#pragma unroll
-        for (int v = 0; v < 4; v++) o[4 * v * LTP + 16 * c] = res[q][h][c][v];
+        for (int v = 0; v < 4; v++) {
+          o[4 * v * LTP] = D10[v];
+          o[4 * v * LTP + 16] = D11[v];
+        }
+      }
     }
-  }
-  __syncthreads();
-  auto out_at = [&](int j, int l) {
-    if (j < tlen[l]) out[toff[l] + (size_t)j * lm.es] = sm[(j & 1) * NPP * LTP + (j >> 1) * LTP + l];
-  };
+    __syncthreads();
 #pragma unroll
-  for (int q = 0; q < 32; q++) {
-    const int j = tile_j(q);
-    if (j < NE) out_at(j, tile_l(q));
+    for (int q = 0; q < 16; q++) {
+      const int c = tile_c(q), l = tile_l(q), j = 2 * c + p;
+      if (c < NPP && j < tlen[l]) out[toff[l] + (size_t)j * lm.es] = sm[c * LTP + l];
+    }
+    __syncthreads();  // the tile is free for the next parity
+    if (p == 0) load(1, vt);
   }
 }
 
-// the tile [2][NPP][LTP], then the lines' offsets (size_t) and lengths (int)
+// the tile [NPP][LTP], then the lines' offsets (size_t) and lengths (int)
 static size_t dense_shared_bytes(int NPP) {
-  return 8 * (size_t)2 * NPP * kDenseLTP + kDenseLT * (sizeof(size_t) + sizeof(int));
+  return 8 * (size_t)NPP * kDenseLTP + kDenseLT * (sizeof(size_t) + sizeof(int));
 }
 
 // Batched lines through a direct plan: dense MFMA products when the plan has
