@@ -1540,7 +1540,14 @@ __global__ void __launch_bounds__(256) k_direct(const double *__restrict__ in,
 // owns row tiles w and NR-1-w, so every wave runs the same number of k-steps;
 // results go back through the LDS tile so that the stores keep the loads'
 // coalescing.
+//
+// CONTIG (lines with unit element stride): lanes run along element pairs
+// (2c, 2c+1), loaded and stored as one 16-byte access; the odd parity's
+// inputs wait in registers during the even parity's products, and the even
+// parity's outputs wait in registers during the odd one's.  Strided lines
+// (lanes along the lines) load and store one parity at a time.
 constexpr int kDenseLT = 32, kDenseLTP = 36, kDenseNT = 256, kDenseKG = 4;
+template <bool CONTIG>
 __global__ void __launch_bounds__(kDenseNT) __attribute__((amdgpu_waves_per_eu(4)))
     k_lines_dense(const double *__restrict__ in, double *__restrict__ out, LineMap lm,
                   size_t nlines, int N, int NPP, const double *__restrict__ K) {
@@ -1549,7 +1556,7 @@ __global__ void __launch_bounds__(kDenseNT) __attribute__((amdgpu_waves_per_eu(4
   const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
   const size_t l0 = (size_t)blockIdx.x * kDenseLT;
   const int LTP = kDenseLTP;
-  const bool contig = lm.es == 1;
+  constexpr bool contig = CONTIG;
   // start and length of the tile's lines, computed once (line_off divides)
   size_t *toff = (size_t *)(sm + NPP * LTP);
   int *tlen = (int *)(toff + kDenseLT);
@@ -1565,11 +1572,17 @@ __global__ void __launch_bounds__(kDenseNT) __attribute__((amdgpu_waves_per_eu(4
   // words of one parity
   auto tile_c = [&](int q) { return contig ? lane + 64 * (q & 1) : 2 * wave + (lane >> 5) + 8 * q; };
   auto tile_l = [&](int q) { return contig ? wave + 4 * (q >> 1) : (lane & 31); };
+  // strided lines: one line per thread, rows c0 + 8 q, so one base address
+  // and a constant step (16 elements) serve all 16 elements
+  const int c0 = 2 * wave + (lane >> 5), ls = lane & 31;
   auto load = [&](int p, double *vt) {
+    const double *src = in + toff[ls] + (size_t)(2 * c0 + p) * lm.es;
+    const size_t step = 16 * lm.es;
+    const int len = tlen[ls];
 #pragma unroll
     for (int q = 0; q < 16; q++) {
-      const int c = tile_c(q), l = tile_l(q), j = 2 * c + p;
-      vt[q] = c < NPP && j < tlen[l] ? in[toff[l] + (size_t)j * lm.es] : 0.0;
+      const int c = c0 + 8 * q;
+      vt[q] = c < NPP && 2 * c + p < len ? src[q * step] : 0.0;
     }
   };
   const int NR = NPP >> 4, npair = (NR + 1) >> 1;
@@ -1578,13 +1591,34 @@ __global__ void __launch_bounds__(kDenseNT) __attribute__((amdgpu_waves_per_eu(4
   const bool two = active && R1 != R0;
   const int nk0 = active ? (NPP - 16 * R0) >> 2 : 0, nk1 = two ? (NPP - 16 * R1) >> 2 : 0;
   const double *Bp = sm + (lane >> 4) * LTP + (lane & 15);
-  double vt[16];
-  load(0, vt);
+  // contiguous lines: element pair (2c, 2c+1) of a line as one 16-byte
+  // access when it is aligned and whole, else two guarded words
+  auto pair_at = [&](int q, double &e0, double &e1) {
+    const int c = tile_c(q), l = tile_l(q), len = tlen[l];
+    const size_t a = toff[l] + 2 * (size_t)c;
+    e0 = e1 = 0.0;
+    if (c >= NPP) return;
+    if (2 * c + 1 < len && !(a & 1)) {
+      const double2 t = *reinterpret_cast<const double2 *>(in + a);
+      e0 = t.x;
+      e1 = t.y;
+    } else {
+      if (2 * c < len) e0 = in[a];
+      if (2 * c + 1 < len) e1 = in[a + 1];
+    }
+  };
+  double vt[16], hold[16];
+  if (CONTIG) {
+#pragma unroll
+    for (int q = 0; q < 16; q++) pair_at(q, hold[q], vt[q]);  // hold: parity 0 for now
+  } else {
+    load(0, vt);
+  }
   for (int p = 0; p < 2; p++) {
 #pragma unroll
     for (int q = 0; q < 16; q++) {
       const int c = tile_c(q);
-      if (c < NPP) sm[c * LTP + tile_l(q)] = vt[q];
+      if (c < NPP) sm[c * LTP + tile_l(q)] = CONTIG && p == 0 ? hold[q] : vt[q];
     }
     __syncthreads();
     d4 D00 = {0.0, 0.0, 0.0, 0.0}, D01 = D00, D10 = D00, D11 = D00;
@@ -1644,13 +1678,37 @@ __global__ void __launch_bounds__(kDenseNT) __attribute__((amdgpu_waves_per_eu(4
       }
     }
     __syncthreads();
+    if (CONTIG) {
 #pragma unroll
-    for (int q = 0; q < 16; q++) {
-      const int c = tile_c(q), l = tile_l(q), j = 2 * c + p;
-      if (c < NPP && j < tlen[l]) out[toff[l] + (size_t)j * lm.es] = sm[c * LTP + l];
+      for (int q = 0; q < 16; q++) {
+        const int c = tile_c(q), l = tile_l(q);
+        const double v = c < NPP ? sm[c * LTP + l] : 0.0;
+        if (p == 0) {
+          hold[q] = v;  // even outputs, stored with the odd ones
+          continue;
+        }
+        const int len = tlen[l];
+        const size_t a = toff[l] + 2 * (size_t)c;
+        if (c >= NPP || 2 * c >= len) continue;
+        if (2 * c + 1 < len && !(a & 1)) {
+          *reinterpret_cast<double2 *>(out + a) = make_double2(hold[q], v);
+        } else {
+          out[a] = hold[q];
+          if (2 * c + 1 < len) out[a + 1] = v;
+        }
+      }
+    } else {
+      double *dst = out + toff[ls] + (size_t)(2 * c0 + p) * lm.es;
+      const size_t step = 16 * lm.es;
+      const int len = tlen[ls];
+#pragma unroll
+      for (int q = 0; q < 16; q++) {
+        const int c = c0 + 8 * q;
+        if (c < NPP && 2 * c + p < len) dst[q * step] = sm[c * LTP + ls];
+      }
     }
     __syncthreads();  // the tile is free for the next parity
-    if (p == 0) load(1, vt);
+    if (!CONTIG && p == 0) load(1, vt);
   }
 }
 
@@ -1666,8 +1724,12 @@ static int launch_direct_lines(const direct_plan_s *dp, const double *d_in, doub
   const int N = (int)dp->N;
   if (dp->d_K[dir] && nlines >= 4) {
     const size_t nb = (nlines + kDenseLT - 1) / kDenseLT;
-    k_lines_dense<<<dim3((unsigned)nb), kDenseNT, dense_shared_bytes(dp->NPP), st>>>(
-        d_in, d_out, lm, nlines, N, dp->NPP, dp->d_K[dir]);
+    if (lm.es == 1)
+      k_lines_dense<true><<<dim3((unsigned)nb), kDenseNT, dense_shared_bytes(dp->NPP), st>>>(
+          d_in, d_out, lm, nlines, N, dp->NPP, dp->d_K[dir]);
+    else
+      k_lines_dense<false><<<dim3((unsigned)nb), kDenseNT, dense_shared_bytes(dp->NPP), st>>>(
+          d_in, d_out, lm, nlines, N, dp->NPP, dp->d_K[dir]);
   } else {
     k_direct<<<dim3((N + 255) / 256, (unsigned)nlines), 256, 0, st>>>(d_in, d_out, lm, N, (int)dir,
                                                                       dp->d_a, dp->d_an, dp->d_dn);
