@@ -374,9 +374,45 @@ def bench_dct(amd, orc, args, steps):
                       "transforms_s": round(c_b / eb, 1),
                       "algorithmic_GBps": round(16 * n_b * c_b / eb / 1e9, 1),
                       "max_rel_err_vs_oracle_rows": errb}
+    # a 256^3 field: DCT-II along axes 2, 1, 0 (65,536 lines each, one launch per axis)
+    n = 256
+    fx = amd.splitmix_uniform(n ** 3, 12)
+    pf = amd.DCTPlan(n)
+    fa, fb = amd.DeviceBuffer(n ** 3), amd.DeviceBuffer(n ** 3)
+    fa.upload(fx)
+    maps = {2: (n * n, n * n, n, 0, 1), 1: (n * n, n, 1, n * n, n), 0: (n * n, n * n, 1, 0, n * n)}
+
+    def field_step():
+        src, dst = fa, fb
+        for ax in (2, 1, 0):
+            pf.execute_lines_device(src.ptr, dst.ptr, 2, *maps[ax])
+            src, dst = dst, src
+
+    # spot check of the axis-2 pass on two lines
+    pf.execute_lines_device(fa.ptr, fb.ptr, 2, *maps[2])
+    pf.synchronize()
+    a2 = fb.download().reshape(n, n, n)
+    errf = max(relerr_(a2[i, j], orc.redft(fx.reshape(n, n, n)[i, j], 2)) for i, j in ((0, 0), (n - 1, 7)))
+    for _ in range(3):
+        field_step()
+    pf.synchronize()
+    sf = max(5, steps // 20)
+    t0 = time.perf_counter()
+    for _ in range(sf):
+        field_step()
+    pf.synchronize()
+    ef = (time.perf_counter() - t0) / sf
+    out["field256"] = {"passes": 3, "ms_per_3_passes": round(1e3 * ef, 4),
+                       "lines_s": round(3 * n * n / ef, 1),
+                       "algorithmic_GBps": round(3 * 16 * n ** 3 / ef / 1e9, 1),
+                       "max_rel_err_vs_oracle_lines": errf}
     out["tol"] = 1e-15 * math.log2(2 * N)
     log(f"[bench] dct: {out}")
     return out
+
+
+def relerr_(x, ref):
+    return float(np.max(np.abs(x - ref)) / np.max(np.abs(ref)))
 
 
 def main():
@@ -489,7 +525,8 @@ def main():
                 "dct": dc, "cpu_baseline": cpu}
         print(json.dumps(line), flush=True)
         if not (dc["max_rel_err_vs_oracle"] <= dc["tol"]
-                and dc["batched"]["max_rel_err_vs_oracle_rows"] <= 1e-15 * math.log2(2048)):
+                and dc["batched"]["max_rel_err_vs_oracle_rows"] <= 1e-15 * math.log2(2048)
+                and dc["field256"]["max_rel_err_vs_oracle_lines"] <= 1e-15 * math.log2(512)):
             log("[bench] FAILED: DCT error above tolerance")
             sys.exit(1)
         return
@@ -728,7 +765,8 @@ def main():
     if batched is not None and not batched["max_rel_err_vs_oracle"] <= 1e-14 * math.log2(8192):
         bad = True
     if dct is not None and not (dct["max_rel_err_vs_oracle"] <= dct["tol"]
-                                and dct["batched"]["max_rel_err_vs_oracle_rows"] <= 1e-15 * math.log2(2048)):
+                                and dct["batched"]["max_rel_err_vs_oracle_rows"] <= 1e-15 * math.log2(2048)
+                                and dct["field256"]["max_rel_err_vs_oracle_lines"] <= 1e-15 * math.log2(512)):
         bad = True
     if bad:
         log(f"[bench] FAILED: error above {tol:.2e}")

@@ -222,6 +222,15 @@ int l2c_execute_dct_device(l2c_dct_plan *p, const double *d_in, double *d_out, i
 int l2c_execute_dct(l2c_dct_plan *p, const double *in, double *out, int kind);
 /* Wait for all work on the plan's device. */
 int l2c_dct_synchronize(l2c_dct_plan *p);
+/* The DCT of `nlines` lines of length N (N <= 4096) in one launch, with the
+ * line map of l2c_execute_lines_device: line l starts at
+ * (l / n0) * line_stride1 + (l % n0) * line_stride0 and its elements are
+ * elem_stride apart (the values <-> coefficients step along one axis of a
+ * field).  The plan's count is not used.  Device pointers, in place allowed,
+ * no synchronisation.  Returns 0 or -1. */
+int l2c_execute_dct_lines_device(l2c_dct_plan *p, const double *d_in, double *d_out, int kind,
+                                 size_t nlines, size_t n0, size_t line_stride0,
+                                 size_t line_stride1, size_t elem_stride, void *stream);
 
 /* Per-kernel timing with HIP events recorded on the launch stream around
  * the execute kernels.  l2c_profile(plan, n) resets the sums and, for n >= 1,

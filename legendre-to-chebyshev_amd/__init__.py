@@ -136,6 +136,8 @@ def lib():
         L.l2c_execute_dct.argtypes = [vp, vp, vp, C.c_int]
         L.l2c_dct_synchronize.restype = C.c_int
         L.l2c_dct_synchronize.argtypes = [vp]
+        L.l2c_execute_dct_lines_device.restype = C.c_int
+        L.l2c_execute_dct_lines_device.argtypes = [vp, vp, vp, C.c_int, _sz, _sz, _sz, _sz, _sz, vp]
         _lib = L
     return _lib
 
@@ -442,6 +444,13 @@ class DCTPlan:
     def synchronize(self):
         if lib().l2c_dct_synchronize(self._plan) != 0:
             raise L2CError(last_error())
+
+    def execute_lines_device(self, d_in, d_out, kind, nlines, n0, ls0, ls1, es, stream=None):
+        """Lines of length N (<= 4096) with the line map of
+        FMMPlan.execute_lines_device, one launch."""
+        if lib().l2c_execute_dct_lines_device(self._plan, d_in, d_out, kind, nlines, n0, ls0, ls1,
+                                              es, stream) != 0:
+            raise L2CError("l2c_execute_dct_lines_device: " + last_error())
 
     def __del__(self):
         if getattr(self, "_plan", None):

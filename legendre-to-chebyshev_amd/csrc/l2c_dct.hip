@@ -238,52 +238,91 @@ __device__ __forceinline__ void pass_root_load(const Tabs &tb, size_t t, double2
 // ---- DCT-II (REDFT10) --------------------------------------------------
 // Post-processing of output index k (0 <= k < M) from Z_k, Zc =
 // conj(Z_{(M-k) mod M}), wk = w^k and w4 = w^{4k}: writes y[k] and y[N-k]
-// (y[0] and y[M] for k = 0).
+// (y[0] and y[M] for k = 0), elements es apart.
 __device__ __forceinline__ void dct2_post(double *y, size_t k, size_t M, double2 Zk, double2 Zc,
-                                          double2 wk, double2 w4) {
+                                          double2 wk, double2 w4, size_t es = 1) {
   if (k == 0) {
     y[0] = 2 * (Zk.x + Zk.y);
-    y[M] = 1.41421356237309504880 * (Zk.x - Zk.y);
+    y[M * es] = 1.41421356237309504880 * (Zk.x - Zk.y);
     return;
   }
   const double2 e = cadd(Zk, Zc);
   const double2 d = csub(Zk, Zc);
   const double2 o = make_double2(d.y, -d.x);  // -i (Z_k - Zc)
   const double2 W = cmul(wk, cadd(e, cmul(w4, o)));
-  y[k] = W.x;
-  y[2 * M - k] = -W.y;
+  y[k * es] = W.x;
+  y[(2 * M - k) * es] = -W.y;
+}
+
+// Item id of a one-pass work-group -> (vector vi, index i < W).  Unit-stride
+// vectors run i fastest (neighbouring lanes, neighbouring words; pairs of
+// words as 16-byte accesses); strided lines run the vector fastest
+// (neighbouring lanes, neighbouring lines).
+template <int W, int NV, bool STRIDED>
+__device__ __forceinline__ void item(int id, int &vi, int &i) {
+  if (STRIDED) {
+    vi = id % NV;
+    i = id / NV;
+  } else {
+    vi = id / W;
+    i = id % W;
+  }
+}
+
+// The start of each of the work-group's lines (line_off divides), in LDS.
+template <int NV, int T>
+__device__ __forceinline__ void line_starts(size_t *loff, const LineMap &lm, size_t v0,
+                                            size_t count) {
+#pragma unroll
+  for (int it = 0; it < (NV + T - 1) / T; ++it) {
+    const int v = threadIdx.x + it * T;
+    if (v < NV) loff[v] = v0 + v < count ? line_off(lm, v0 + v) : 0;
+  }
 }
 
 // One pass for M = 2^LOGM <= 2048: NV = 2048 / M vectors per work-group.
-template <int LOGM, int NV>
+// Vector v (v < count) starts at line_off(lm, v), its elements lm.es apart
+// (STRIDED), or 1 apart at an even offset (otherwise).
+template <int LOGM, int NV, bool STRIDED>
 __global__ void __launch_bounds__(kElems / 8)
-    k_dct2_1p(const double *__restrict__ x, double *__restrict__ y, size_t count, Tabs tb) {
+    k_dct2_1p(const double *__restrict__ x, double *__restrict__ y, size_t count, LineMap lm,
+              Tabs tb) {
   constexpr int M = 1 << LOGM, LD = M + 1, H = M / 2, T = kElems / 8;
-  constexpr size_t N = 2 * size_t(M);
   constexpr int NI = NV * M / T, NL = NV * H / T;  // items per thread: post, load
   extern __shared__ double2 z[];
   double2 *ts = z + NV * LD;
-  const size_t v0 = size_t(blockIdx.x) * NV;
+  size_t *loff = reinterpret_cast<size_t *>(ts + (LOGM > 3 ? M / 4 : 0));
+  const size_t v0 = size_t(blockIdx.x) * NV, es = STRIDED ? lm.es : 1;
+  line_starts<NV, T>(loff, lm, v0, count);
   double2 t1[NI], t4[NI], p0[NL], p1[NL];
 #pragma unroll
   for (int it = 0; it < NI; ++it) {
-    const int k = (threadIdx.x + it * T) % M;
+    int vi, k;
+    item<M, NV, STRIDED>(threadIdx.x + it * T, vi, k);
     t1[it] = tb.g1[k];
     t4[it] = tb.g4[k];
   }
+  load_quarter<LOGM, T>(ts, tb.tw);
+  __syncthreads();  // line starts
 #pragma unroll
   for (int it = 0; it < NL; ++it) {
-    const int id = threadIdx.x + it * T, vi = id / H, m = id % H;
+    int vi, m;
+    item<H, NV, STRIDED>(threadIdx.x + it * T, vi, m);
     if (v0 + vi < count) {
-      const double *src = x + (v0 + vi) * N + 4 * size_t(m);
-      p0[it] = *reinterpret_cast<const double2 *>(src);
-      p1[it] = *reinterpret_cast<const double2 *>(src + 2);
+      const double *src = x + loff[vi] + 4 * size_t(m) * es;
+      if (STRIDED) {
+        p0[it] = make_double2(src[0], src[es]);
+        p1[it] = make_double2(src[2 * es], src[3 * es]);
+      } else {
+        p0[it] = *reinterpret_cast<const double2 *>(src);
+        p1[it] = *reinterpret_cast<const double2 *>(src + 2);
+      }
     }
   }
-  load_quarter<LOGM, T>(ts, tb.tw);
 #pragma unroll
   for (int it = 0; it < NL; ++it) {
-    const int id = threadIdx.x + it * T, vi = id / H, m = id % H;
+    int vi, m;
+    item<H, NV, STRIDED>(threadIdx.x + it * T, vi, m);
     if (v0 + vi < count) {
       z[vi * LD + m] = make_double2(p0[it].x, p1[it].x);
       z[vi * LD + (M - 1 - m)] = make_double2(p1[it].y, p0[it].y);
@@ -293,10 +332,11 @@ __global__ void __launch_bounds__(kElems / 8)
   fft_cols<LOGM, NV, T, -1>(z, ts);
 #pragma unroll
   for (int it = 0; it < NI; ++it) {
-    const int id = threadIdx.x + it * T, vi = id / M, k = id % M;
+    int vi, k;
+    item<M, NV, STRIDED>(threadIdx.x + it * T, vi, k);
     if (v0 + vi < count)
-      dct2_post(y + (v0 + vi) * N, k, M, z[vi * LD + k],
-                conj2(z[vi * LD + ((M - k) & (M - 1))]), t1[it], t4[it]);
+      dct2_post(y + loff[vi], k, M, z[vi * LD + k], conj2(z[vi * LD + ((M - k) & (M - 1))]),
+                t1[it], t4[it], es);
   }
 }
 
@@ -404,58 +444,75 @@ __device__ __forceinline__ double2 dct3_pre(double a, double b, double c, double
   return make_double2(s.x - t.y, s.y + t.x);
 }
 
-template <int LOGM, int NV>
+template <int LOGM, int NV, bool STRIDED>
 __global__ void __launch_bounds__(kElems / 8)
-    k_dct3_1p(const double *__restrict__ Y, double *__restrict__ x, size_t count, Tabs tb) {
+    k_dct3_1p(const double *__restrict__ Y, double *__restrict__ x, size_t count, LineMap lm,
+              Tabs tb) {
   constexpr int M = 1 << LOGM, LD = M + 1, H = M / 2, N = 2 * M, T = kElems / 8;
   constexpr int NI = NV * M / T, NL = NV * N / T, NS = NV * H / T;
   extern __shared__ double2 z[];
   double2 *ts = z + NV * LD;
+  size_t *loff = reinterpret_cast<size_t *>(ts + (LOGM > 3 ? M / 4 : 0));
   double *yr = reinterpret_cast<double *>(z);  // column stride 2 LD doubles
-  const size_t v0 = size_t(blockIdx.x) * NV;
+  const size_t v0 = size_t(blockIdx.x) * NV, es = STRIDED ? lm.es : 1;
+  line_starts<NV, T>(loff, lm, v0, count);
   double2 t1[NI], t4[NI];
   double q[NL];
 #pragma unroll
   for (int it = 0; it < NI; ++it) {
-    const int k = (threadIdx.x + it * T) % M;
+    int vi, k;
+    item<M, NV, STRIDED>(threadIdx.x + it * T, vi, k);
     t1[it] = tb.g1[k];
     t4[it] = tb.g4[k];
   }
-#pragma unroll
-  for (int it = 0; it < NL; ++it) {
-    const int id = threadIdx.x + it * T, vi = id / N;
-    if (v0 + vi < count) q[it] = Y[(v0 + vi) * N + id % N];
-  }
   load_quarter<LOGM, T>(ts, tb.tw);
+  __syncthreads();  // line starts
 #pragma unroll
   for (int it = 0; it < NL; ++it) {
-    const int id = threadIdx.x + it * T, vi = id / N;
-    if (v0 + vi < count) yr[vi * 2 * LD + id % N] = q[it];
+    int vi, r;
+    item<N, NV, STRIDED>(threadIdx.x + it * T, vi, r);
+    if (v0 + vi < count) q[it] = Y[loff[vi] + size_t(r) * es];
+  }
+#pragma unroll
+  for (int it = 0; it < NL; ++it) {
+    int vi, r;
+    item<N, NV, STRIDED>(threadIdx.x + it * T, vi, r);
+    if (v0 + vi < count) yr[vi * 2 * LD + r] = q[it];
   }
   __syncthreads();
   double2 zp[NI];
 #pragma unroll
   for (int it = 0; it < NI; ++it) {
-    const int id = threadIdx.x + it * T, vi = id / M, k = id % M;
+    int vi, k;
+    item<M, NV, STRIDED>(threadIdx.x + it * T, vi, k);
     const double *c = yr + vi * 2 * LD;
     zp[it] = dct3_pre(c[k], k ? c[N - k] : 0.0, c[k + M], c[M - k], t1[it], t4[it]);
   }
   __syncthreads();
 #pragma unroll
   for (int it = 0; it < NI; ++it) {
-    const int id = threadIdx.x + it * T;
-    z[(id / M) * LD + id % M] = zp[it];
+    int vi, k;
+    item<M, NV, STRIDED>(threadIdx.x + it * T, vi, k);
+    z[vi * LD + k] = zp[it];
   }
   __syncthreads();
   fft_cols<LOGM, NV, T, 1>(z, ts);
 #pragma unroll
   for (int it = 0; it < NS; ++it) {
-    const int id = threadIdx.x + it * T, vi = id / H, m = id % H;
+    int vi, m;
+    item<H, NV, STRIDED>(threadIdx.x + it * T, vi, m);
     if (v0 + vi < count) {
       const double2 A = z[vi * LD + m], B = z[vi * LD + (M - 1 - m)];
-      double *dst = x + (v0 + vi) * N + 4 * size_t(m);
-      *reinterpret_cast<double2 *>(dst) = make_double2(A.x, B.y);
-      *reinterpret_cast<double2 *>(dst + 2) = make_double2(A.y, B.x);
+      double *dst = x + loff[vi] + 4 * size_t(m) * es;
+      if (STRIDED) {
+        dst[0] = A.x;
+        dst[es] = B.y;
+        dst[2 * es] = A.y;
+        dst[3 * es] = B.x;
+      } else {
+        *reinterpret_cast<double2 *>(dst) = make_double2(A.x, B.y);
+        *reinterpret_cast<double2 *>(dst + 2) = make_double2(A.y, B.x);
+      }
     }
   }
 }
@@ -613,23 +670,24 @@ __global__ void __launch_bounds__(NC * (1 << LOGL) / 8)
   }
 }
 
-// N = 1 and N = 2.
+// N = 1 and N = 2, one vector per thread.
 __global__ void k_dct_tiny(const double *__restrict__ in, double *__restrict__ out, size_t count,
-                           int N, int kind) {
+                           LineMap lm, int N, int kind) {
   const size_t v = size_t(blockIdx.x) * blockDim.x + threadIdx.x;
   if (v >= count) return;
   constexpr double r2 = 1.41421356237309504880;
+  const size_t o = line_off(lm, v), es = lm.es;
   if (N == 1) {
-    out[v] = kind == 2 ? 2 * in[v] : in[v];
+    out[o] = kind == 2 ? 2 * in[o] : in[o];
     return;
   }
-  const double a = in[2 * v], b = in[2 * v + 1];
+  const double a = in[o], b = in[o + es];
   if (kind == 2) {
-    out[2 * v] = 2 * (a + b);
-    out[2 * v + 1] = r2 * (a - b);
+    out[o] = 2 * (a + b);
+    out[o + es] = r2 * (a - b);
   } else {
-    out[2 * v] = a + r2 * b;
-    out[2 * v + 1] = a - r2 * b;
+    out[o] = a + r2 * b;
+    out[o + es] = a - r2 * b;
   }
 }
 
@@ -692,16 +750,20 @@ namespace l2c {
 namespace {
 
 template <int LOGM>
-void launch_1p(const l2c_dct_plan *p, const double *in, double *out, int kind, hipStream_t st) {
+void launch_1p(const l2c_dct_plan *p, const double *in, double *out, int kind, size_t count,
+               const LineMap &lm, bool strided, hipStream_t st) {
   constexpr int M = 1 << LOGM, NV = kElems / M;
-  const size_t shmem = (size_t(NV) * (M + 1) + (LOGM > 3 ? M / 4 : 0)) * sizeof(double2);
-  const dim3 grid(unsigned((p->count + NV - 1) / NV));
-  if (kind == 2)
-    hipLaunchKernelGGL((k_dct2_1p<LOGM, NV>), grid, dim3(kElems / 8), shmem, st, in, out,
-                       p->count, p->tb);
+  const size_t shmem =
+      (size_t(NV) * (M + 1) + (LOGM > 3 ? M / 4 : 0)) * sizeof(double2) + NV * sizeof(size_t);
+  const dim3 grid(unsigned((count + NV - 1) / NV)), blk(kElems / 8);
+  if (kind == 2 && strided)
+    hipLaunchKernelGGL((k_dct2_1p<LOGM, NV, true>), grid, blk, shmem, st, in, out, count, lm, p->tb);
+  else if (kind == 2)
+    hipLaunchKernelGGL((k_dct2_1p<LOGM, NV, false>), grid, blk, shmem, st, in, out, count, lm, p->tb);
+  else if (strided)
+    hipLaunchKernelGGL((k_dct3_1p<LOGM, NV, true>), grid, blk, shmem, st, in, out, count, lm, p->tb);
   else
-    hipLaunchKernelGGL((k_dct3_1p<LOGM, NV>), grid, dim3(kElems / 8), shmem, st, in, out,
-                       p->count, p->tb);
+    hipLaunchKernelGGL((k_dct3_1p<LOGM, NV, false>), grid, blk, shmem, st, in, out, count, lm, p->tb);
 }
 
 // Columns per work-group for a pass of FFT length 2^LOGL: E elements, and at
@@ -797,26 +859,37 @@ int dispatch_pass(int logl, const l2c_dct_plan *p, int pass, const double *in, d
 }
 
 template <int LOG>
-int dispatch_1p(const l2c_dct_plan *p, const double *in, double *out, int kind, hipStream_t st) {
+int dispatch_1p(const l2c_dct_plan *p, const double *in, double *out, int kind, size_t count,
+                const LineMap &lm, bool strided, hipStream_t st) {
   if constexpr (LOG > kOnePassLogM) {
     set_error("dct: one-pass length out of range");
     return -1;
   } else {
     if (p->logM == LOG) {
-      launch_1p<LOG>(p, in, out, kind, st);
+      launch_1p<LOG>(p, in, out, kind, count, lm, strided, st);
       return 0;
     }
-    return dispatch_1p<LOG + 1>(p, in, out, kind, st);
+    return dispatch_1p<LOG + 1>(p, in, out, kind, count, lm, strided, st);
   }
 }
 
-int run_dct(const l2c_dct_plan *p, const double *in, double *out, int kind, hipStream_t st) {
+// Lines of a one-pass plan (N <= 4096): line l starts at line_off(lm, l).
+int run_dct_lines(const l2c_dct_plan *p, const double *in, double *out, int kind, size_t nlines,
+                  const LineMap &lm, bool strided, hipStream_t st) {
   if (p->N <= 2) {
-    const unsigned blocks = unsigned((p->count + 255) / 256);
-    hipLaunchKernelGGL(k_dct_tiny, dim3(blocks), dim3(256), 0, st, in, out, p->count, int(p->N),
-                       kind);
-  } else if (p->logM <= kOnePassLogM) {
-    if (dispatch_1p<1>(p, in, out, kind, st)) return -1;
+    const unsigned blocks = unsigned((nlines + 255) / 256);
+    hipLaunchKernelGGL(k_dct_tiny, dim3(blocks), dim3(256), 0, st, in, out, nlines, lm,
+                       int(p->N), kind);
+  } else if (dispatch_1p<1>(p, in, out, kind, nlines, lm, strided, st)) {
+    return -1;
+  }
+  L2C_HIP(hipGetLastError(), -1);
+  return 0;
+}
+
+int run_dct(const l2c_dct_plan *p, const double *in, double *out, int kind, hipStream_t st) {
+  if (p->logM <= kOnePassLogM) {
+    return run_dct_lines(p, in, out, kind, p->count, LineMap{p->count, p->N, 0, 1}, false, st);
   } else {
     if (dispatch_pass<kMinPassLog>(p->l_first, p, 1, in, out, kind, st)) return -1;
     if (p->l_mid && dispatch_mid<kMinPassLog>(p, kind, st)) return -1;
@@ -969,6 +1042,31 @@ int l2c_execute_dct_device(l2c_dct_plan *p, const double *d_in, double *d_out, i
     return -1;
   }
   return run_dct(p, d_in, d_out, kind, static_cast<hipStream_t>(stream));
+}
+
+int l2c_execute_dct_lines_device(l2c_dct_plan *p, const double *d_in, double *d_out, int kind,
+                                 size_t nlines, size_t n0, size_t ls0, size_t ls1, size_t es,
+                                 void *stream) {
+  clear_error();
+  if (!p || !d_in || !d_out || nlines == 0 || n0 == 0 || es == 0) {
+    set_error("l2c_execute_dct_lines_device: invalid argument");
+    return -1;
+  }
+  if (kind != 2 && kind != 3) {
+    set_error("l2c_execute_dct_lines_device: kind must be 2 (REDFT10) or 3 (REDFT01)");
+    return -1;
+  }
+  if (p->N > 2 && p->logM > kOnePassLogM) {
+    set_error("l2c_execute_dct_lines_device: lines longer than 4096 are not supported "
+              "(use l2c_execute_dct_device on contiguous vectors)");
+    return -1;
+  }
+  // unit-stride lines at even offsets of 16-byte aligned arrays move element
+  // pairs as 16-byte accesses; anything else takes the strided kernels
+  const bool strided = es != 1 || (ls0 & 1) || (nlines > n0 && (ls1 & 1)) || !aligned16(d_in) ||
+                       !aligned16(d_out);
+  return run_dct_lines(p, d_in, d_out, kind, nlines, LineMap{n0, ls0, ls1, es}, strided,
+                       static_cast<hipStream_t>(stream));
 }
 
 int l2c_execute_dct(l2c_dct_plan *p, const double *in, double *out, int kind) {

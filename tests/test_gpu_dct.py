@@ -131,3 +131,111 @@ def test_errors(amd):
     assert "kind" in amd.last_error()
     assert lib.l2c_execute_dct_device(plan._plan, buf.ptr + 8, buf.ptr, 2, None) == -1
     assert "aligned" in amd.last_error()
+
+
+# ---- lines: the DCT along one axis of a field (l2c_execute_dct_lines_device)
+
+def axis_map(shape, axis):
+    """(nlines, n0, ls0, ls1, es) of the lines of a C-ordered array along axis."""
+    es = int(np.prod(shape[axis + 1:]))
+    outer = int(np.prod(shape[:axis]))
+    return outer * es, es, 1, shape[axis] * es, es
+
+
+def dct_along(amd, x, axis, kind, inplace=False):
+    N = x.shape[axis]
+    plan = amd.DCTPlan(N)
+    din = amd.DeviceBuffer(x.size)
+    dout = din if inplace else amd.DeviceBuffer(x.size)
+    din.upload(x.ravel())
+    plan.execute_lines_device(din.ptr, dout.ptr, kind, *axis_map(x.shape, axis))
+    plan.synchronize()
+    return dout.download().reshape(x.shape)
+
+
+@pytest.mark.parametrize("shape", [(3, 5, 64), (4, 256, 6), (1024, 2, 3), (2, 3, 4096), (7, 1, 2)])
+@pytest.mark.parametrize("kind", [2, 3])
+def test_lines_along_every_axis(amd, orc, shape, kind):
+    x = np.random.default_rng(sum(shape) + kind).uniform(-1, 1, shape)
+    for axis in range(3):
+        N = shape[axis]
+        if N & (N - 1):
+            continue
+        announce(f"dct lines kind={kind} shape={shape} axis={axis}")
+        y = dct_along(amd, x, axis, kind)
+        ref = np.moveaxis(orc.redft(np.ascontiguousarray(np.moveaxis(x, axis, -1)), kind), -1, axis)
+        err = max(relerr(a, b) for a, b in zip(np.moveaxis(y, axis, -1).reshape(-1, N),
+                                                np.moveaxis(ref, axis, -1).reshape(-1, N)))
+        assert err < tol(N), (axis, err)
+
+
+@pytest.mark.parametrize("N", [1, 2, 8, 512])
+def test_lines_odd_offsets_and_in_place(amd, orc, N):
+    """Unit-stride lines at odd offsets (a padded row stride N + 1) take the
+    strided kernels; in place along a strided axis."""
+    announce(f"dct lines odd offsets N={N}")
+    nl = 37
+    x = np.random.default_rng(N).uniform(-1, 1, nl * (N + 1))
+    plan = amd.DCTPlan(N)
+    din, dout = amd.DeviceBuffer(x.size), amd.DeviceBuffer(x.size)
+    din.upload(x)
+    dout.upload(np.zeros(x.size))
+    plan.execute_lines_device(din.ptr, dout.ptr, 2, nl, nl, N + 1, 0, 1)
+    plan.synchronize()
+    y = dout.download().reshape(nl, N + 1)
+    xr = x.reshape(nl, N + 1)
+    for r in (0, 1, nl - 1):
+        assert relerr(y[r, :N], orc.redft(xr[r, :N], 2)) < tol(N)
+    assert np.all(y[:, N] == 0)  # the padding is not touched
+    z = np.random.default_rng(N + 1).uniform(-1, 1, (N, 3))
+    got = dct_along(amd, z, 0, 3, inplace=True)
+    assert relerr(got[:, 1], orc.redft(z[:, 1], 3)) < tol(N)
+
+
+def test_field_values_pipeline(amd):
+    """A 3-D Legendre series a_ijk -> values at the Chebyshev-Gauss grid
+    (L2C, then DCT-III along every axis) equals the series evaluated there
+    directly; values -> DCT-II along every axis, scaling, C2L gives a back."""
+    from numpy.polynomial import legendre
+    n = 32
+    announce(f"dct field pipeline n={n}")
+    rng = np.random.default_rng(9)
+    k = np.arange(n)
+    a = rng.uniform(-1, 1, (n, n, n)) / np.multiply.outer(np.multiply.outer(k + 1, k + 1), k + 1) ** 1.5
+    fp = amd.FMMPlan(n, 64, 18, amd.BOTH)
+    dct = amd.DCTPlan(n)
+    bufs = [amd.DeviceBuffer(n ** 3) for _ in range(2)]
+
+    def along(arr, fn):
+        bufs[0].upload(arr.ravel())
+        for ax in (2, 1, 0):
+            fn(bufs[0].ptr, bufs[1].ptr, *axis_map((n, n, n), ax))
+            bufs[0], bufs[1] = bufs[1], bufs[0]
+        fp.synchronize()
+        dct.synchronize()
+        return bufs[0].download().reshape(n, n, n)
+
+    c = along(a, lambda i, o, *m: fp.execute_lines_device(i, o, amd.L2C, *m))
+    # per axis: values = REDFT01 of (c_0, c_1/2, ...); c = REDFT10(values)/n, c_0 halved
+    hv, hc = np.where(k == 0, 1.0, 0.5), np.where(k == 0, 0.5, 1.0)
+    c = c * np.multiply.outer(np.multiply.outer(hv, hv), hv)
+    f = along(c, lambda i, o, *m: dct.execute_lines_device(i, o, 3, *m))
+    x = np.cos(np.pi * (2 * k + 1) / (2 * n))
+    V = legendre.legvander(x, n - 1)  # V[j, l] = P_l(x_j)
+    f_ref = np.einsum("il,jm,kn,lmn->ijk", V, V, V, a)
+    assert relerr(f, f_ref) < 1e-13
+    g = along(f, lambda i, o, *m: dct.execute_lines_device(i, o, 2, *m))
+    g = g / n ** 3 * np.multiply.outer(np.multiply.outer(hc, hc), hc)
+    back = along(g, lambda i, o, *m: fp.execute_lines_device(i, o, amd.C2L, *m))
+    assert relerr(back, a) < 1e-13
+
+
+def test_lines_errors(amd):
+    plan = amd.DCTPlan(1 << 13)
+    buf = amd.DeviceBuffer(1 << 13)
+    lib = amd.lib()
+    assert lib.l2c_execute_dct_lines_device(plan._plan, buf.ptr, buf.ptr, 2, 1, 1, 0, 0, 1, None) == -1
+    assert "4096" in amd.last_error()
+    small = amd.DCTPlan(64)
+    assert lib.l2c_execute_dct_lines_device(small._plan, buf.ptr, buf.ptr, 2, 0, 1, 0, 0, 1, None) == -1
+    assert lib.l2c_execute_dct_lines_device(small._plan, buf.ptr, buf.ptr, 5, 1, 1, 0, 0, 1, None) == -1
