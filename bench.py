@@ -344,7 +344,7 @@ def bench_dct(amd, orc, args, steps):
     ref = orc.redft(x, 2)
     err = float(np.max(np.abs(dout.download() - ref)) / np.max(np.abs(ref)))
     logm = int(math.log2(N)) - 1
-    passes = 1 if logm <= 11 else (2 if logm < 16 else 3)
+    passes = 1 if logm <= 11 else (2 if logm < 23 else 3)
     out = {"N": N, "kind": "REDFT10", "us": round(1e6 * el, 3),
            "value": round(1 / el, 1), "unit": "transforms/s",
            "vs_fftw_published": round(1 / el / FFTW_REDFT10_2_20["value"], 1) if N == 1 << 20 else None,

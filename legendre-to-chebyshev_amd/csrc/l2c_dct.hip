@@ -26,11 +26,10 @@
 //   first pass: columns c < Cf (index m = c + Cf n), FFT of length Lf over n,
 //               times e^{-+2 pi i c k / M}, rows of the workspace W[c][k];
 //   last pass:  the length-Cf DFTs over c, output index Cq k1 + column.
-// For M < 2^16 the last pass is one FFT of length Lq = Cf on Cq = Lf columns
-// (two passes).  From M = 2^16 on, Cf = L1 L2 is split once more (three
-// passes, each of length <= 256): k_fft_mid runs the FFTs of length L2 and
-// the last pass those of length Lq = L1 on Cq = L2 Lf columns.  Short passes
-// leave every work-group enough columns to touch whole 128-byte row segments.
+// For M < 2^23 the last pass is one FFT of length Lq = Cf on Cq = Lf columns
+// (two passes, lengths up to 2048).  From M = 2^23 on, Cf = L1 L2 is split
+// once more (three passes of length <= 256): k_fft_mid runs the FFTs of
+// length L2 and the last pass those of length Lq = L1 on Cq = L2 Lf columns.
 // Each work-group of the first and last pass owns a chunk of columns together
 // with the chunk that the DCT's index symmetry pairs with it, so every pre-
 // and post-processing step finds both of its operands in LDS.  DCT-II pairs
@@ -55,8 +54,9 @@ namespace {
 
 constexpr int kTwLog = 12;        // in-FFT roots e^{-2 pi i u / 4096}
 constexpr int kOnePassLogM = 11;  // M <= 2048 in one pass
-constexpr int kThreePassLogM = 16;  // M >= 2^16 in three passes, else two
-constexpr int kMinPassLog = 5, kMaxPassLog = 8;  // pass lengths 32 ... 256
+constexpr int kThreePassLogM = 23;  // M >= 2^23 in three passes, else two
+constexpr int kMinPassLog = 5, kMaxPassLog = 11;  // pass lengths 32 ... 2048
+constexpr int kMaxMidLog = 8;  // three-pass plans split 2^23, 2^24 as 7/8/8, 8/8/8
 constexpr int kMaxLogM = 24;        // three passes of at most 256 each
 constexpr int kElems = 2048;      // elements per work-group (at least two columns)
 
@@ -280,10 +280,14 @@ __device__ __forceinline__ void line_starts(size_t *loff, const LineMap &lm, siz
   }
 }
 
+// Layouts of the one-pass kernels' vectors: contiguous (vector v at v N),
+// lines with unit element stride at even offsets, and strided lines.
+enum { kVecs = 0, kLinesUnit = 1, kLinesStrided = 2 };
+
 // One pass for M = 2^LOGM <= 2048: NV = 2048 / M vectors per work-group.
-// Vector v (v < count) starts at line_off(lm, v), its elements lm.es apart
-// (STRIDED), or 1 apart at an even offset (otherwise).
-template <int LOGM, int NV, bool STRIDED>
+// Vector v (v < count) starts at v N (kVecs) or line_off(lm, v) (lines), its
+// elements lm.es apart (kLinesStrided) or 1 apart.
+template <int LOGM, int NV, int LAY>
 __global__ void __launch_bounds__(kElems / 8)
     k_dct2_1p(const double *__restrict__ x, double *__restrict__ y, size_t count, LineMap lm,
               Tabs tb) {
@@ -292,8 +296,13 @@ __global__ void __launch_bounds__(kElems / 8)
   extern __shared__ double2 z[];
   double2 *ts = z + NV * LD;
   size_t *loff = reinterpret_cast<size_t *>(ts + (LOGM > 3 ? M / 4 : 0));
+  constexpr bool STRIDED = LAY == kLinesStrided;
   const size_t v0 = size_t(blockIdx.x) * NV, es = STRIDED ? lm.es : 1;
-  line_starts<NV, T>(loff, lm, v0, count);
+  if constexpr (LAY != kVecs) {
+    line_starts<NV, T>(loff, lm, v0, count);
+    lds_sync();  // line starts; then every global load is issued before the first LDS store
+  }
+  auto start = [&](int vi) -> size_t { return LAY == kVecs ? (v0 + vi) * (2 * size_t(M)) : loff[vi]; };
   double2 t1[NI], t4[NI], p0[NL], p1[NL];
 #pragma unroll
   for (int it = 0; it < NI; ++it) {
@@ -302,14 +311,12 @@ __global__ void __launch_bounds__(kElems / 8)
     t1[it] = tb.g1[k];
     t4[it] = tb.g4[k];
   }
-  load_quarter<LOGM, T>(ts, tb.tw);
-  __syncthreads();  // line starts
 #pragma unroll
   for (int it = 0; it < NL; ++it) {
     int vi, m;
     item<H, NV, STRIDED>(threadIdx.x + it * T, vi, m);
     if (v0 + vi < count) {
-      const double *src = x + loff[vi] + 4 * size_t(m) * es;
+      const double *src = x + start(vi) + 4 * size_t(m) * es;
       if (STRIDED) {
         p0[it] = make_double2(src[0], src[es]);
         p1[it] = make_double2(src[2 * es], src[3 * es]);
@@ -319,6 +326,7 @@ __global__ void __launch_bounds__(kElems / 8)
       }
     }
   }
+  load_quarter<LOGM, T>(ts, tb.tw);
 #pragma unroll
   for (int it = 0; it < NL; ++it) {
     int vi, m;
@@ -335,7 +343,7 @@ __global__ void __launch_bounds__(kElems / 8)
     int vi, k;
     item<M, NV, STRIDED>(threadIdx.x + it * T, vi, k);
     if (v0 + vi < count)
-      dct2_post(y + loff[vi], k, M, z[vi * LD + k], conj2(z[vi * LD + ((M - k) & (M - 1))]),
+      dct2_post(y + start(vi), k, M, z[vi * LD + k], conj2(z[vi * LD + ((M - k) & (M - 1))]),
                 t1[it], t4[it], es);
   }
 }
@@ -444,7 +452,7 @@ __device__ __forceinline__ double2 dct3_pre(double a, double b, double c, double
   return make_double2(s.x - t.y, s.y + t.x);
 }
 
-template <int LOGM, int NV, bool STRIDED>
+template <int LOGM, int NV, int LAY>
 __global__ void __launch_bounds__(kElems / 8)
     k_dct3_1p(const double *__restrict__ Y, double *__restrict__ x, size_t count, LineMap lm,
               Tabs tb) {
@@ -454,8 +462,13 @@ __global__ void __launch_bounds__(kElems / 8)
   double2 *ts = z + NV * LD;
   size_t *loff = reinterpret_cast<size_t *>(ts + (LOGM > 3 ? M / 4 : 0));
   double *yr = reinterpret_cast<double *>(z);  // column stride 2 LD doubles
+  constexpr bool STRIDED = LAY == kLinesStrided;
   const size_t v0 = size_t(blockIdx.x) * NV, es = STRIDED ? lm.es : 1;
-  line_starts<NV, T>(loff, lm, v0, count);
+  if constexpr (LAY != kVecs) {
+    line_starts<NV, T>(loff, lm, v0, count);
+    lds_sync();  // line starts; then every global load is issued before the first LDS store
+  }
+  auto start = [&](int vi) -> size_t { return LAY == kVecs ? (v0 + vi) * (2 * size_t(M)) : loff[vi]; };
   double2 t1[NI], t4[NI];
   double q[NL];
 #pragma unroll
@@ -465,14 +478,13 @@ __global__ void __launch_bounds__(kElems / 8)
     t1[it] = tb.g1[k];
     t4[it] = tb.g4[k];
   }
-  load_quarter<LOGM, T>(ts, tb.tw);
-  __syncthreads();  // line starts
 #pragma unroll
   for (int it = 0; it < NL; ++it) {
     int vi, r;
     item<N, NV, STRIDED>(threadIdx.x + it * T, vi, r);
-    if (v0 + vi < count) q[it] = Y[loff[vi] + size_t(r) * es];
+    if (v0 + vi < count) q[it] = Y[start(vi) + size_t(r) * es];
   }
+  load_quarter<LOGM, T>(ts, tb.tw);
 #pragma unroll
   for (int it = 0; it < NL; ++it) {
     int vi, r;
@@ -503,7 +515,7 @@ __global__ void __launch_bounds__(kElems / 8)
     item<H, NV, STRIDED>(threadIdx.x + it * T, vi, m);
     if (v0 + vi < count) {
       const double2 A = z[vi * LD + m], B = z[vi * LD + (M - 1 - m)];
-      double *dst = x + loff[vi] + 4 * size_t(m) * es;
+      double *dst = x + start(vi) + 4 * size_t(m) * es;
       if (STRIDED) {
         dst[0] = A.x;
         dst[es] = B.y;
@@ -749,21 +761,28 @@ struct l2c_dct_plan_s {
 namespace l2c {
 namespace {
 
-template <int LOGM>
-void launch_1p(const l2c_dct_plan *p, const double *in, double *out, int kind, size_t count,
-               const LineMap &lm, bool strided, hipStream_t st) {
+template <int LOGM, int LAY>
+void launch_1p_lay(const l2c_dct_plan *p, const double *in, double *out, int kind, size_t count,
+                   const LineMap &lm, hipStream_t st) {
   constexpr int M = 1 << LOGM, NV = kElems / M;
   const size_t shmem =
       (size_t(NV) * (M + 1) + (LOGM > 3 ? M / 4 : 0)) * sizeof(double2) + NV * sizeof(size_t);
   const dim3 grid(unsigned((count + NV - 1) / NV)), blk(kElems / 8);
-  if (kind == 2 && strided)
-    hipLaunchKernelGGL((k_dct2_1p<LOGM, NV, true>), grid, blk, shmem, st, in, out, count, lm, p->tb);
-  else if (kind == 2)
-    hipLaunchKernelGGL((k_dct2_1p<LOGM, NV, false>), grid, blk, shmem, st, in, out, count, lm, p->tb);
-  else if (strided)
-    hipLaunchKernelGGL((k_dct3_1p<LOGM, NV, true>), grid, blk, shmem, st, in, out, count, lm, p->tb);
+  if (kind == 2)
+    hipLaunchKernelGGL((k_dct2_1p<LOGM, NV, LAY>), grid, blk, shmem, st, in, out, count, lm, p->tb);
   else
-    hipLaunchKernelGGL((k_dct3_1p<LOGM, NV, false>), grid, blk, shmem, st, in, out, count, lm, p->tb);
+    hipLaunchKernelGGL((k_dct3_1p<LOGM, NV, LAY>), grid, blk, shmem, st, in, out, count, lm, p->tb);
+}
+
+template <int LOGM>
+void launch_1p(const l2c_dct_plan *p, const double *in, double *out, int kind, size_t count,
+               const LineMap &lm, int lay, hipStream_t st) {
+  if (lay == kVecs)
+    launch_1p_lay<LOGM, kVecs>(p, in, out, kind, count, lm, st);
+  else if (lay == kLinesUnit)
+    launch_1p_lay<LOGM, kLinesUnit>(p, in, out, kind, count, lm, st);
+  else
+    launch_1p_lay<LOGM, kLinesStrided>(p, in, out, kind, count, lm, st);
 }
 
 // Columns per work-group for a pass of FFT length 2^LOGL: E elements, and at
@@ -826,7 +845,7 @@ void launch_pass(const l2c_dct_plan *p, int pass, const double *in, double *out,
 
 template <int LOG>
 int dispatch_mid(const l2c_dct_plan *p, int kind, hipStream_t st) {
-  if constexpr (LOG > kMaxPassLog) {
+  if constexpr (LOG > kMaxMidLog) {
     set_error("dct: middle pass length out of range");
     return -1;
   } else {
@@ -860,27 +879,27 @@ int dispatch_pass(int logl, const l2c_dct_plan *p, int pass, const double *in, d
 
 template <int LOG>
 int dispatch_1p(const l2c_dct_plan *p, const double *in, double *out, int kind, size_t count,
-                const LineMap &lm, bool strided, hipStream_t st) {
+                const LineMap &lm, int lay, hipStream_t st) {
   if constexpr (LOG > kOnePassLogM) {
     set_error("dct: one-pass length out of range");
     return -1;
   } else {
     if (p->logM == LOG) {
-      launch_1p<LOG>(p, in, out, kind, count, lm, strided, st);
+      launch_1p<LOG>(p, in, out, kind, count, lm, lay, st);
       return 0;
     }
-    return dispatch_1p<LOG + 1>(p, in, out, kind, count, lm, strided, st);
+    return dispatch_1p<LOG + 1>(p, in, out, kind, count, lm, lay, st);
   }
 }
 
 // Lines of a one-pass plan (N <= 4096): line l starts at line_off(lm, l).
 int run_dct_lines(const l2c_dct_plan *p, const double *in, double *out, int kind, size_t nlines,
-                  const LineMap &lm, bool strided, hipStream_t st) {
+                  const LineMap &lm, int lay, hipStream_t st) {
   if (p->N <= 2) {
     const unsigned blocks = unsigned((nlines + 255) / 256);
     hipLaunchKernelGGL(k_dct_tiny, dim3(blocks), dim3(256), 0, st, in, out, nlines, lm,
                        int(p->N), kind);
-  } else if (dispatch_1p<1>(p, in, out, kind, nlines, lm, strided, st)) {
+  } else if (dispatch_1p<1>(p, in, out, kind, nlines, lm, lay, st)) {
     return -1;
   }
   L2C_HIP(hipGetLastError(), -1);
@@ -889,7 +908,7 @@ int run_dct_lines(const l2c_dct_plan *p, const double *in, double *out, int kind
 
 int run_dct(const l2c_dct_plan *p, const double *in, double *out, int kind, hipStream_t st) {
   if (p->logM <= kOnePassLogM) {
-    return run_dct_lines(p, in, out, kind, p->count, LineMap{p->count, p->N, 0, 1}, false, st);
+    return run_dct_lines(p, in, out, kind, p->count, LineMap{p->count, p->N, 0, 1}, kVecs, st);
   } else {
     if (dispatch_pass<kMinPassLog>(p->l_first, p, 1, in, out, kind, st)) return -1;
     if (p->l_mid && dispatch_mid<kMinPassLog>(p, kind, st)) return -1;
@@ -1065,8 +1084,8 @@ int l2c_execute_dct_lines_device(l2c_dct_plan *p, const double *d_in, double *d_
   // pairs as 16-byte accesses; anything else takes the strided kernels
   const bool strided = es != 1 || (ls0 & 1) || (nlines > n0 && (ls1 & 1)) || !aligned16(d_in) ||
                        !aligned16(d_out);
-  return run_dct_lines(p, d_in, d_out, kind, nlines, LineMap{n0, ls0, ls1, es}, strided,
-                       static_cast<hipStream_t>(stream));
+  return run_dct_lines(p, d_in, d_out, kind, nlines, LineMap{n0, ls0, ls1, es},
+                       strided ? kLinesStrided : kLinesUnit, static_cast<hipStream_t>(stream));
 }
 
 int l2c_execute_dct(l2c_dct_plan *p, const double *in, double *out, int kind) {

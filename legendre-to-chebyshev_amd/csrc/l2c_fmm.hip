@@ -56,15 +56,6 @@ size_t ws_doubles(const Geo &g) { return ws_layout(g).total; }
 __host__ __device__ __forceinline__ int sk4(int c) { return c + (c >> 2); }
 __host__ __device__ __forceinline__ int sk8(int k) { return k + (k >> 3); }
 
-// Work-group barrier for LDS traffic only: it orders LDS reads and writes
-// like __syncthreads() but does not wait for global loads and stores still
-// in flight, so prefetches and result stores overlap the next phase.  Use it
-// only where the threads exchange data through LDS alone.
-__device__ __forceinline__ void lds_sync() {
-  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup", "local");
-  __builtin_amdgcn_s_barrier();
-  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup", "local");
-}
 
 // ---------------------------------------------------------------------------
 // k_up: grid (2^(lu+2), lines), 256 threads.  G = 2^gu <= 16 leaf chunks.

@@ -88,6 +88,16 @@ __host__ __device__ inline size_t slot_base(int level) { return ((size_t)4 << le
 // Batches of uneven lengths set off (device array: start of line l) and n
 // (device array: length of line l <= the plan's N); the kernels then read
 // zeros past a line's end and store only its first n[l] outputs.
+// Work-group barrier for LDS traffic only: it orders LDS reads and writes
+// like __syncthreads() but does not wait for global loads and stores still
+// in flight, so prefetches and result stores overlap the next phase.  Use it
+// only where the threads exchange data through LDS alone.
+__device__ __forceinline__ void lds_sync() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup", "local");
+  __builtin_amdgcn_s_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup", "local");
+}
+
 struct LineMap {
   size_t n0, ls0, ls1, es;
   const size_t *off = nullptr;

@@ -54,8 +54,7 @@ def test_one_pass_sizes(amd, orc, kind, logN):
 @pytest.mark.parametrize("kind", [2, 3])
 @pytest.mark.parametrize("logN", [14, 15, 16, 17, 18, 19, 20, 21])
 def test_multi_pass_sizes(amd, orc, kind, logN):
-    """N = 2^14 ... 2^16: two HBM passes (lengths 64 ... 256); N = 2^17 ...
-    2^21: three passes (lengths 32 ... 128)."""
+    """N = 2^14 ... 2^21: two HBM passes of lengths 64 ... 1024."""
     N = 1 << logN
     check(amd, orc, N, 1, kind, 100 + logN + kind)
     if logN <= 17:
@@ -66,15 +65,14 @@ def test_multi_pass_sizes(amd, orc, kind, logN):
 @pytest.mark.parametrize("logN,count", [(13, 1024), (16, 64), (17, 32), (20, 4)])
 def test_wide_work_groups(amd, orc, kind, logN, count):
     """Jobs of at least 2^21 complex points run the passes with 4096 elements
-    per work-group (two passes at N = 2^13, three from N = 2^17 on, the middle
-    one wide from N = 2^18 on)."""
+    per work-group (pass lengths 64 ... 1024)."""
     check(amd, orc, 1 << logN, count, kind, 300 + logN + kind)
 
 
 @pytest.mark.parametrize("kind", [2, 3])
 def test_largest_pass_lengths(amd, kind):
-    """N = 2^23 (the reference's largest timed size) and 2^25 (three passes
-    of 256, the largest plan) against scipy's pocketfft."""
+    """N = 2^23 (the reference's largest timed size: two passes of 2048) and
+    2^25 (three passes of 256, the largest plan) against scipy's pocketfft."""
     sf = pytest.importorskip("scipy.fft")
     for logN in (23, 25):
         N = 1 << logN
