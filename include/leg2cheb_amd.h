@@ -250,8 +250,9 @@ void *l2c_malloc_device(size_t bytes);
 int l2c_free_device(void *p);
 int l2c_memcpy_h2d(void *dst, const void *src, size_t bytes);
 int l2c_memcpy_d2h(void *dst, const void *src, size_t bytes);
-/* Device-to-device copy (a plain copy kernel of known traffic: the bench
- * scales its HBM counters by it). */
+/* Device-to-device copy, complete when the call returns (a plain copy kernel
+ * of known traffic: the bench scales its HBM counters by it, and a copy
+ * larger than the 256 MB last-level cache evicts a plan's tables). */
 int l2c_memcpy_d2d(void *dst, const void *src, size_t bytes);
 
 #pragma GCC visibility pop

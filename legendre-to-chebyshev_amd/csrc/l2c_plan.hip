@@ -985,6 +985,7 @@ int l2c_memcpy_d2h(void *dst, const void *src, size_t bytes) {
 }
 int l2c_memcpy_d2d(void *dst, const void *src, size_t bytes) {
   L2C_HIP(hipMemcpy(dst, src, bytes, hipMemcpyDeviceToDevice), -1);
+  L2C_HIP(hipStreamSynchronize(nullptr), -1);  // complete on return
   return 0;
 }
 int l2c_synchronize(fmm_plan *p) {
