@@ -59,6 +59,7 @@ constexpr int kMinPassLog = 5, kMaxPassLog = 11;  // pass lengths 32 ... 2048
 constexpr int kMaxMidLog = 8;  // three-pass plans split 2^23, 2^24 as 7/8/8, 8/8/8
 constexpr int kMaxLogM = 24;        // three passes of at most 256 each
 constexpr int kElems = 2048;      // elements per work-group (at least two columns)
+constexpr size_t kMaxGridY = 65535;  // vectors per launch of the multi-pass kernels
 
 // Device twiddle tables of a plan; w = e^{-2 pi i / (4N)}.
 struct Tabs {
@@ -910,9 +911,17 @@ int run_dct(const l2c_dct_plan *p, const double *in, double *out, int kind, hipS
   if (p->logM <= kOnePassLogM) {
     return run_dct_lines(p, in, out, kind, p->count, LineMap{p->count, p->N, 0, 1}, kVecs, st);
   } else {
-    if (dispatch_pass<kMinPassLog>(p->l_first, p, 1, in, out, kind, st)) return -1;
-    if (p->l_mid && dispatch_mid<kMinPassLog>(p, kind, st)) return -1;
-    if (dispatch_pass<kMinPassLog>(p->l_last, p, 2, in, out, kind, st)) return -1;
+    // the passes put the vectors in grid.y: at most kMaxGridY per launch,
+    // each chunk reusing the front of the workspace
+    for (size_t v0 = 0; v0 < p->count; v0 += kMaxGridY) {
+      l2c_dct_plan_s q = *p;
+      q.count = std::min<size_t>(kMaxGridY, p->count - v0);
+      const double *ci = in + v0 * p->N;
+      double *co = out + v0 * p->N;
+      if (dispatch_pass<kMinPassLog>(q.l_first, &q, 1, ci, co, kind, st)) return -1;
+      if (q.l_mid && dispatch_mid<kMinPassLog>(&q, kind, st)) return -1;
+      if (dispatch_pass<kMinPassLog>(q.l_last, &q, 2, ci, co, kind, st)) return -1;
+    }
   }
   L2C_HIP(hipGetLastError(), -1);
   return 0;

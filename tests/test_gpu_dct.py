@@ -70,6 +70,20 @@ def test_wide_work_groups(amd, orc, kind, logN, count):
 
 
 @pytest.mark.parametrize("kind", [2, 3])
+def test_more_vectors_than_one_launch(amd, orc, kind):
+    """65,537 vectors of N = 8192 (4.3 GB each way): the two-pass kernels take
+    at most 65,535 vectors per launch, so the call runs two chunks; the
+    vectors either side of the boundary and the last one against the oracle."""
+    N, count = 8192, 65537
+    announce(f"dct kind={kind} N={N} count={count} (two launches per pass)")
+    x = np.random.default_rng(500 + kind).random((count, N)) - 0.5
+    y = run(amd, x, N, count, kind)
+    for r in (0, 65534, 65535, 65536):
+        assert relerr(y[r], orc.redft(x[r], kind)) < tol(N), r
+    del x, y
+
+
+@pytest.mark.parametrize("kind", [2, 3])
 def test_largest_pass_lengths(amd, kind):
     """N = 2^23 (the reference's largest timed size: two passes of 2048) and
     2^25 (three passes of 256, the largest plan) against scipy's pocketfft."""
