@@ -1900,26 +1900,26 @@ int run_lines(fmm_plan_s *p, const double *d_in, double *d_out, size_t dir, size
       prof_end(dp->prof, st);
       return rc;
     }
+    // lines go in grid.y: at most 65535 per launch
     for (size_t l0 = 0; l0 < nlines; l0 += 65535) {
       const size_t nl = std::min<size_t>(65535, nlines - l0);
-      if (l0 % lm.n0 != 0 && nlines > 65535) {
-        set_error("run_lines: batches above 65535 lines need n0 to divide 65535-line chunks");
-        return -1;
-      }
-      const size_t base = (l0 / lm.n0) * lm.ls1 + (l0 % lm.n0) * lm.ls0;
+      lm.l0 = l0;
       prof_begin(dp->prof, K_DIRECT, st);
       k_direct<<<dim3((N + 255) / 256, (unsigned)nl), 256, 0, st>>>(
-          d_in + base, d_out + base, lm, N, (int)dir, dp->d_a, dp->d_an, dp->d_dn);
+          d_in, d_out, lm, N, (int)dir, dp->d_a, dp->d_an, dp->d_dn);
       prof_end(dp->prof, st);
       L2C_HIP(hipGetLastError(), -1);
     }
     return 0;
   }
-  if (nlines > 65535) {
-    set_error("run_lines: at most 65535 lines per call for FMM plans");
-    return -1;
+  // lines go in grid.y: at most 65535 per launch, each chunk reusing the
+  // front of the plan's line workspace
+  for (size_t l0 = 0; l0 < nlines; l0 += 65535) {
+    lm.l0 = l0;
+    if (launch_fmm(p, d_in, d_out, dir, lm, std::min<size_t>(65535, nlines - l0), st, 0))
+      return -1;
   }
-  return launch_fmm(p, d_in, d_out, dir, lm, nlines, st, 0);
+  return 0;
 }
 
 }  // namespace l2c

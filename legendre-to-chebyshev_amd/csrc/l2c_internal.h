@@ -83,11 +83,6 @@ __host__ __device__ inline size_t total_sub(size_t L) { return 3 * total_blocks(
 // Chunk slots of the multipole store: level l holds 2^(l+2) chunks.
 __host__ __device__ inline size_t slot_base(int level) { return ((size_t)4 << level) - 4; }
 
-// Line l of a batch starts at (l / n0) * ls1 + (l % n0) * ls0; its element i
-// is at that offset + i * es.  A single transform is {1, 0, 0, stride}.
-// Batches of uneven lengths set off (device array: start of line l) and n
-// (device array: length of line l <= the plan's N); the kernels then read
-// zeros past a line's end and store only its first n[l] outputs.
 // Work-group barrier for LDS traffic only: it orders LDS reads and writes
 // like __syncthreads() but does not wait for global loads and stores still
 // in flight, so prefetches and result stores overlap the next phase.  Use it
@@ -98,13 +93,21 @@ __device__ __forceinline__ void lds_sync() {
   __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup", "local");
 }
 
+// Line l of a batch starts at (l / n0) * ls1 + (l % n0) * ls0; its element i
+// is at that offset + i * es.  A single transform is {1, 0, 0, stride}.
+// Batches of uneven lengths set off (device array: start of line l) and n
+// (device array: length of line l <= the plan's N); the kernels then read
+// zeros past a line's end and store only its first n[l] outputs.
 struct LineMap {
   size_t n0, ls0, ls1, es;
   const size_t *off = nullptr;
   const int *n = nullptr;
+  // first line of this launch in the (n0, ls0, ls1) numbering, for batches
+  // split over several launches (packed tables are offset by pointer instead)
+  size_t l0 = 0;
 };
 __host__ __device__ inline size_t line_off(const LineMap &m, size_t l) {
-  return m.off ? m.off[l] : (l / m.n0) * m.ls1 + (l % m.n0) * m.ls0;
+  return m.off ? m.off[l] : ((m.l0 + l) / m.n0) * m.ls1 + ((m.l0 + l) % m.n0) * m.ls0;
 }
 __host__ __device__ inline int line_len(const LineMap &m, size_t l, int N) {
   return m.n ? m.n[l] : N;

@@ -96,3 +96,24 @@ def test_field_round_trip(amd):
     for axis in (0, 1, 2):
         y = along_axis(amd, y, axis, 1)
     assert float(np.max(np.abs(y - x))) < 1e-10
+
+
+@pytest.mark.parametrize("N,maxs", [(512, 64), (600, 256), (200, 64)])
+def test_more_lines_than_one_launch(amd, orc, N, maxs):
+    """65,541 lines along the middle axis of a (9363, N, 7) array in one call:
+    the FMM (N = 512) and one-row-per-thread direct (N = 600, maxs = 256)
+    kernels take at most 65,535 lines per launch, so the call runs two, the
+    second starting inside a group of n0 = 7 lines (65,535 = 7 * 9362 + 1);
+    the dense kernel (N = 200) takes them all in one.  The lines either side
+    of the boundary, the first and the last against the oracle."""
+    P, Q = 9363, 7
+    announce(f"lines across launches N={N} maxs={maxs} nlines={P * Q}")
+    x = np.random.default_rng(N).random((P, N, Q)) - 0.5
+    op = orc.FMMPlan(N, maxs, 18, orc.BOTH)
+    for d in (0, 1):
+        g = along_axis(amd, x, 1, d, maxs=maxs)
+        for line in (0, 65533, 65534, 65535, 65536, P * Q - 1):
+            p, q = divmod(line, Q)
+            ref = op.execute(np.ascontiguousarray(x[p, :, q]), d)
+            assert relerr(g[p, :, q], ref) <= tol(N), (d, line)
+        del g
