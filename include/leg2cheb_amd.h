@@ -137,10 +137,11 @@ int l2c_get_plan_info(const fmm_plan *plan, l2c_plan_info *info);
 int l2c_execute_device(fmm_plan *plan, const double *d_in, double *d_out,
                        size_t direction, size_t stride, void *stream);
 
-/* Many lines of length N through one plan in one launch.  Line l (l < nlines)
- * has first element at (l / n0) * line_stride1 + (l % n0) * line_stride0 and
- * its elements at that offset + i * elem_stride.  Device pointers.  With
- * n0 = nlines, line_stride0 = distance between consecutive lines. */
+/* Many lines of length N through one plan in one call (one launch sequence
+ * per 65,535 lines; any nlines).  Line l (l < nlines) has first element at
+ * (l / n0) * line_stride1 + (l % n0) * line_stride0 and its elements at that
+ * offset + i * elem_stride.  Device pointers.  With n0 = nlines,
+ * line_stride0 = distance between consecutive lines. */
 int l2c_execute_lines_device(fmm_plan *plan, const double *d_in, double *d_out,
                              size_t direction, size_t nlines, size_t n0,
                              size_t line_stride0, size_t line_stride1,
