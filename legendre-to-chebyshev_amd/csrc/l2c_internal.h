@@ -174,7 +174,7 @@ struct direct_plan_s {
   size_t N = 0, direction = 0, precompute = 0;
   int device = 0;
   double *d_a = nullptr, *d_an = nullptr, *d_dn = nullptr;
-  // dense parity blocks for batched lines (N <= kDenseMaxN): per direction
+  // dense parity blocks for batched lines (N <= kGemmMaxN): per direction
   // 2 x NPP x NPP, NPP = ceil(N/2) rounded up to 16
   double *d_K[2] = {nullptr, nullptr};
   int NPP = 0;
@@ -212,6 +212,9 @@ struct fmm_plan_2d_s {
 namespace l2c {
 // batched lines of the direct path up to this length run as dense MFMA products
 constexpr int kDenseMaxN = 256;
+// direct plans up to this length build the dense parity blocks; batched lines
+// with kDenseMaxN < N <= kGemmMaxN run them as a tiled product
+constexpr int kGemmMaxN = 4096;
 // implemented in l2c_exec.hip
 int run_fmm(fmm_plan_s *p, const double *d_in, double *d_out, size_t dir, size_t stride,
             hipStream_t st, int band_only);

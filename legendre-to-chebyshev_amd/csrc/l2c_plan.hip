@@ -460,7 +460,7 @@ direct_plan_s *make_direct(size_t N, size_t direction, size_t precompute, bool s
     destroy_direct(dp);
     return nullptr;
   }
-  if (N <= (size_t)kDenseMaxN) {
+  if (N <= (size_t)kGemmMaxN) {
     dp->NPP = (int)(((N + 1) / 2 + 15) / 16 * 16);
     const size_t nk = 2 * (size_t)dp->NPP * dp->NPP;
     for (int d = 0; d < 2; d++) {

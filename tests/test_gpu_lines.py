@@ -82,6 +82,25 @@ def test_dense_lengths(amd, orc, N):
         assert per_line_error(g0, np.ascontiguousarray(o1.T), 0) <= tol(N)
 
 
+@pytest.mark.parametrize("N", [257, 300, 511, 512, 513, 1000, 1024, 2047, 2048, 4095, 4096])
+def test_gemm_lengths(amd, orc, N):
+    """Direct-path plans above N = 256 (maxs = N/4 rounded up, so L = 0): the
+    tiled dense product, with row tiles partly past NPP (N = 300, 513, 1000,
+    2047, 4095), a first chunk starting mid-row-tile, and ragged line tiles
+    (70 lines), on contiguous and on strided lines, both directions."""
+    maxs = (N + 3) // 4
+    announce(f"gemm lines N={N} maxs={maxs}")
+    rng = np.random.default_rng(N + 7)
+    x = rng.uniform(-1, 1, (70, N)) / np.sqrt(np.arange(1, N + 1))
+    for d in (0, 1):
+        g1 = along_axis(amd, x, 1, d, maxs=maxs)
+        o1 = oracle_along_axis(orc, x, 1, d, maxs=maxs)
+        assert per_line_error(g1, o1, 1) <= tol(N), d
+        xt = np.ascontiguousarray(x.T)
+        g0 = along_axis(amd, xt, 0, d, maxs=maxs)
+        assert per_line_error(g0, np.ascontiguousarray(o1.T), 0) <= tol(N), d
+
+
 def test_field_round_trip(amd):
     """L2C along the three axes, then C2L back: the input returns
     (src/bin/l2c.c:342, 387 thresholds)."""
