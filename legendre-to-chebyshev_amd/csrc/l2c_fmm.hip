@@ -1710,28 +1710,30 @@ static size_t dense_shared_bytes(int NPP) {
 
 // Batched lines of the direct path with kDenseMaxN < N <= kGemmMaxN: the same
 // parity blocks K_p (NPP up to 2048) as a tiled product on FP64 MFMA.  A
-// work-group owns 64 pair-rows (row tiles 4 rt + wave, one per wave) of both
-// parities for 32 lines and walks the columns c >= 64 rt in chunks of 64:
-// the chunk's input pairs (2c, 2c+1) of its 32 lines are staged in LDS
-// [parity][c][line] (16-byte loads along each line when the lines are
-// contiguous), then each wave runs v_mfma_f64_16x16x4f64 with A = K_p
-// fragments (global memory, L2/MALL-resident, all of the chunk's k-steps
-// issued before its products) and B = the staged rows, two 16-line column
-// tiles per parity.  The results go out through the same LDS tile, each
-// thread storing an (even, odd) output pair of one line.  Grid (line tiles,
-// row tiles): consecutive work-groups share a row tile's K rows.
+// work-group owns 64 pair-rows (row tiles 4 rt + wave, one per wave) of one
+// parity p for 32 lines and walks the columns c >= 64 rt in chunks of 64:
+// the chunk's inputs 2c + p of its 32 lines are staged in LDS [c][line], then
+// each wave runs v_mfma_f64_16x16x4f64 with A = K_p fragments (global
+// memory, L2/MALL-resident) and B = the staged rows, two 16-line column
+// tiles.  The next chunk's inputs and fragments load into registers during
+// the current chunk's products.  The results go out through the same LDS
+// tile.  The two parities are separate work-groups, which halves the longest
+// work-group's chain of chunks (row tile 0 walks all NPP / 64 of them).  Grid
+// (2 x line tiles, row tiles), the parity in the lowest bit of blockIdx.x:
+// the long row tiles of both parities are dispatched first, and consecutive
+// work-groups share a row tile's K rows.
 constexpr int kGemmRT = 64, kGemmLT = 32, kGemmKC = 64, kGemmNT = 256;
 template <bool CONTIG>
-__global__ void __launch_bounds__(kGemmNT) __attribute__((amdgpu_waves_per_eu(2)))
+__global__ void __launch_bounds__(kGemmNT) __attribute__((amdgpu_waves_per_eu(3)))
     k_lines_gemm(const double *__restrict__ in, double *__restrict__ out, LineMap lm,
                  size_t nlines, int N, int NPP, const double *__restrict__ K) {
   extern __shared__ double sm[];
   constexpr int LTP = kDenseLTP, KC = kGemmKC, LT = kGemmLT;
   const int tid = threadIdx.x, lane = tid & 63;
   const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
-  const int rt = blockIdx.y;
-  const size_t l0 = (size_t)blockIdx.x * LT;
-  size_t *toff = (size_t *)(sm + 2 * KC * LTP);
+  const int rt = blockIdx.y, p = blockIdx.x & 1;
+  const size_t l0 = (size_t)(blockIdx.x >> 1) * LT;
+  size_t *toff = (size_t *)(sm + KC * LTP);
   int *tlen = (int *)(toff + LT);
   if (tid < LT) {
     const size_t line = l0 + tid;
@@ -1743,36 +1745,22 @@ __global__ void __launch_bounds__(kGemmNT) __attribute__((amdgpu_waves_per_eu(2)
   const bool active = R < NR;  // the last row tile may be partly past NPP
   const size_t es = CONTIG ? 1 : lm.es;
   const d4 z = {0.0, 0.0, 0.0, 0.0};
-  d4 D00 = z, D01 = z, D10 = z, D11 = z;  // [parity][16-line half]
+  d4 D0 = z, D1 = z;  // 16-line halves
   const double *Bp = sm + (lane >> 4) * LTP + (lane & 15);
-  constexpr int QS = KC * LT / kGemmNT;  // staged pairs per thread per chunk
-  // chunk c0's input pairs of this thread (line idx / KC, column c0 + idx % KC)
-  // and both parities' fragments of this wave, into registers
-  double e0[QS], e1[QS], a0n[KC / 4], a1n[KC / 4];
+  const double *Kp = K + ((size_t)(p * NR + R) * nkt) * 64 + lane;
+  constexpr int QS = KC * LT / kGemmNT;  // staged inputs per thread per chunk
+  // chunk c0's inputs of this thread (line idx / KC, column c0 + idx % KC)
+  // and this wave's fragments, into registers
+  double e[QS], an[KC / 4];
   auto fetch = [&](int c0) {
     const int ks0 = c0 == kGemmRT * rt ? 4 * wave : 0, nks = min(KC, NPP - c0) >> 2;
-    const double *K0 = K + ((size_t)R * nkt + (c0 >> 2)) * 64 + lane;
-    const double *K1 = K0 + (size_t)NR * nkt * 64;
 #pragma unroll
-    for (int ks = 0; ks < KC / 4; ks++) {
-      const bool on = active && ks >= ks0 && ks < nks;
-      a0n[ks] = on ? K0[64 * ks] : 0.0;
-      a1n[ks] = on ? K1[64 * ks] : 0.0;
-    }
+    for (int ks = 0; ks < KC / 4; ks++)
+      an[ks] = active && ks >= ks0 && ks < nks ? Kp[((c0 >> 2) + ks) * 64] : 0.0;
 #pragma unroll
     for (int q = 0; q < QS; q++) {
-      const int idx = q * kGemmNT + tid, l = idx / KC, c = c0 + idx % KC;
-      const int len = tlen[l];
-      const size_t a = toff[l] + 2 * (size_t)c * es;
-      e0[q] = e1[q] = 0.0;
-      if (CONTIG && 2 * c + 1 < len && !(a & 1)) {
-        const double2 t = *reinterpret_cast<const double2 *>(in + a);
-        e0[q] = t.x;
-        e1[q] = t.y;
-      } else {
-        if (2 * c < len) e0[q] = in[a];
-        if (2 * c + 1 < len) e1[q] = in[a + es];
-      }
+      const int idx = q * kGemmNT + tid, l = idx / KC, j = 2 * (c0 + idx % KC) + p;
+      e[q] = j < tlen[l] ? in[toff[l] + (size_t)j * es] : 0.0;
     }
   };
   fetch(kGemmRT * rt);
@@ -1781,18 +1769,14 @@ __global__ void __launch_bounds__(kGemmNT) __attribute__((amdgpu_waves_per_eu(2)
     // chunk (K_p is zero below it), up to NPP in the last
     const int ks0 = c0 == kGemmRT * rt ? 4 * wave : 0;
     const int nks = min(KC, NPP - c0) >> 2;
-    double a0[KC / 4], a1[KC / 4];
+    double a[KC / 4];
 #pragma unroll
     for (int q = 0; q < QS; q++) {
-      const int idx = q * kGemmNT + tid, l = idx / KC, cc = idx % KC;
-      sm[cc * LTP + l] = e0[q];
-      sm[(KC + cc) * LTP + l] = e1[q];
+      const int idx = q * kGemmNT + tid;
+      sm[(idx % KC) * LTP + idx / KC] = e[q];
     }
 #pragma unroll
-    for (int ks = 0; ks < KC / 4; ks++) {
-      a0[ks] = a0n[ks];
-      a1[ks] = a1n[ks];
-    }
+    for (int ks = 0; ks < KC / 4; ks++) a[ks] = an[ks];
     __syncthreads();
     // the next chunk's loads run during this chunk's products
     if (c0 + KC < NPP) fetch(c0 + KC);
@@ -1800,11 +1784,9 @@ __global__ void __launch_bounds__(kGemmNT) __attribute__((amdgpu_waves_per_eu(2)
 #pragma unroll
       for (int ks = 0; ks < KC / 4; ks++) {
         if (ks < ks0 || ks >= nks) continue;
-        const double *b0 = Bp + 4 * ks * LTP, *b1 = b0 + KC * LTP;
-        D00 = __builtin_amdgcn_mfma_f64_16x16x4f64(a0[ks], b0[0], D00, 0, 0, 0);
-        D01 = __builtin_amdgcn_mfma_f64_16x16x4f64(a0[ks], b0[16], D01, 0, 0, 0);
-        D10 = __builtin_amdgcn_mfma_f64_16x16x4f64(a1[ks], b1[0], D10, 0, 0, 0);
-        D11 = __builtin_amdgcn_mfma_f64_16x16x4f64(a1[ks], b1[16], D11, 0, 0, 0);
+        const double *b = Bp + 4 * ks * LTP;
+        D0 = __builtin_amdgcn_mfma_f64_16x16x4f64(a[ks], b[0], D0, 0, 0, 0);
+        D1 = __builtin_amdgcn_mfma_f64_16x16x4f64(a[ks], b[16], D1, 0, 0, 0);
       }
     }
     __syncthreads();  // the tile is free for the next chunk
@@ -1814,31 +1796,21 @@ __global__ void __launch_bounds__(kGemmNT) __attribute__((amdgpu_waves_per_eu(2)
     double *o = sm + (16 * wave + (lane >> 4)) * LTP + (lane & 15);
 #pragma unroll
     for (int v = 0; v < 4; v++) {
-      o[4 * v * LTP] = D00[v];
-      o[4 * v * LTP + 16] = D01[v];
-      o[(KC + 4 * v) * LTP] = D10[v];
-      o[(KC + 4 * v) * LTP + 16] = D11[v];
+      o[4 * v * LTP] = D0[v];
+      o[4 * v * LTP + 16] = D1[v];
     }
   }
   __syncthreads();
 #pragma unroll
   for (int q = 0; q < kGemmRT * LT / kGemmNT; q++) {
     const int idx = q * kGemmNT + tid, l = idx / kGemmRT, rr = idx % kGemmRT;
-    const int r = kGemmRT * rt + rr, len = tlen[l];
-    if (r >= NPP || 2 * r >= len) continue;
-    const double ev = sm[rr * LTP + l], od = sm[(KC + rr) * LTP + l];
-    const size_t a = toff[l] + 2 * (size_t)r * es;
-    if (CONTIG && 2 * r + 1 < len && !(a & 1)) {
-      *reinterpret_cast<double2 *>(out + a) = make_double2(ev, od);
-    } else {
-      out[a] = ev;
-      if (2 * r + 1 < len) out[a + es] = od;
-    }
+    const int r = kGemmRT * rt + rr, i = 2 * r + p;
+    if (r < NPP && i < tlen[l]) out[toff[l] + (size_t)i * es] = sm[rr * LTP + l];
   }
 }
 
 size_t gemm_shared_bytes() {
-  return 8 * (size_t)2 * kGemmKC * kDenseLTP + kGemmLT * (sizeof(size_t) + sizeof(int));
+  return 8 * (size_t)kGemmKC * kDenseLTP + kGemmLT * (sizeof(size_t) + sizeof(int));
 }
 
 // Batched lines through a direct plan: dense MFMA products when the plan has
@@ -1848,7 +1820,7 @@ static int launch_direct_lines(const direct_plan_s *dp, const double *d_in, doub
   const int N = (int)dp->N;
   if (dp->d_K[dir] && nlines >= 4 && N > kDenseMaxN) {
     const size_t nb = (nlines + kGemmLT - 1) / kGemmLT;
-    const dim3 grid((unsigned)nb, (unsigned)((dp->NPP + kGemmRT - 1) / kGemmRT));
+    const dim3 grid((unsigned)(2 * nb), (unsigned)((dp->NPP + kGemmRT - 1) / kGemmRT));
     if (lm.es == 1)
       k_lines_gemm<true><<<grid, kGemmNT, gemm_shared_bytes(), st>>>(d_in, d_out, lm, nlines, N,
                                                                       dp->NPP, dp->d_K[dir]);
