@@ -738,30 +738,38 @@ def main():
         op = orc.FMMPlan(N, maxs, M, d)
         out = np.zeros(N)
         op.execute(u, d, out=out)
-        reps, t0 = 0, time.perf_counter()
+        # each call timed on its own as well, for the min and average of the
+        # reference's own protocol (src/bin/l2c.c:181-212)
+        reps, t0, per = 0, time.perf_counter(), []
         while True:
+            tc = time.perf_counter()
             op.execute(u, d, out=out)
+            per.append(time.perf_counter() - tc)
             reps += 1
             if time.perf_counter() - t0 > args.cpu_seconds:
                 break
         cpu_t = time.perf_counter() - t0
         cpu = {"value": round(reps / cpu_t, 3), "unit": "transforms/s", "cores": nt, "kind": "port",
                "sample": f"oracle FMM execute, N={N} L2C maxs={maxs} M={M}, {reps} repeats "
-                         f"in {cpu_t:.1f} s, OpenMP {nt} threads, own loops (no BLAS)"}
+                         f"in {cpu_t:.1f} s, OpenMP {nt} threads, own loops (no BLAS)",
+               "min_s": min(per), "avg_s": sum(per) / len(per)}
         log(f"[bench] cpu baseline {cpu}")
         # the same restatement on one core (the reference's published figures
         # are single-core)
         orc.set_num_threads(1)
-        reps1, t0 = 0, time.perf_counter()
+        reps1, t0, per1 = 0, time.perf_counter(), []
         while True:
+            tc = time.perf_counter()
             op.execute(u, d, out=out)
+            per1.append(time.perf_counter() - tc)
             reps1 += 1
             if time.perf_counter() - t0 > args.cpu_seconds / 2:
                 break
         t1 = time.perf_counter() - t0
         orc.set_num_threads(nt)
         cpu["one_core"] = {"value": round(reps1 / t1, 3), "unit": "transforms/s", "cores": 1,
-                           "sample": f"{reps1} repeats in {t1:.1f} s"}
+                           "sample": f"{reps1} repeats in {t1:.1f} s",
+                           "min_s": min(per1), "avg_s": sum(per1) / len(per1)}
         log(f"[bench] cpu baseline, one core: {cpu['one_core']}")
 
     line = {
