@@ -771,6 +771,11 @@ def main():
                            "sample": f"{reps1} repeats in {t1:.1f} s",
                            "min_s": min(per1), "avg_s": sum(per1) / len(per1)}
         log(f"[bench] cpu baseline, one core: {cpu['one_core']}")
+        # the same restatement built at -Ofast, all threads
+        r, n, el, mn, av = orc.ofast_rate(N, maxs, M, d, u, args.cpu_seconds / 2, threads=nt)
+        cpu["ofast"] = {"value": round(r, 3), "unit": "transforms/s", "cores": nt,
+                        "sample": f"{n} repeats in {el:.1f} s, gcc -Ofast", "min_s": mn, "avg_s": av}
+        log(f"[bench] cpu baseline, -Ofast: {cpu['ofast']}")
 
     line = {
         "metric": METRIC, "value": round(value, 2), "unit": "transforms/s", "n_gpus": 1,

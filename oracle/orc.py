@@ -254,6 +254,37 @@ def direct_dd_rows(u, rows, direction):
     return out
 
 
+def ofast_rate(N, maxs, M, direction, u, seconds, threads=None):
+    """Transforms/s of the same restatement built at -Ofast (liborc_l2c_ofast.so),
+    for the CPU baseline only: (rate, repeats, seconds, min_s, avg_s)."""
+    import time
+    F = C.CDLL(os.path.join(_HERE, "liborc_l2c_ofast.so"))
+    F.orc_create_fmm.restype = C.POINTER(_Plan)
+    F.orc_create_fmm.argtypes = [_sz] * 7
+    F.orc_free_fmm.argtypes = [C.POINTER(_Plan)]
+    F.orc_execute.restype = _sz
+    F.orc_execute.argtypes = [_dp, _dp, C.POINTER(_Plan), _sz, _sz]
+    F.orc_num_threads.restype = C.c_int
+    F.orc_set_num_threads.argtypes = [C.c_int]
+    if threads:
+        F.orc_set_num_threads(threads)
+    p = F.orc_create_fmm(N, maxs, M, direction, 0, 0, 0)
+    if not p:
+        raise ValueError("orc_create_fmm failed")
+    u = np.ascontiguousarray(u, dtype=np.float64)
+    out = np.zeros_like(u)
+    F.orc_execute(_p(u), _p(out), p, direction, 1)
+    reps, per, t0 = 0, [], time.perf_counter()
+    while time.perf_counter() - t0 < seconds:
+        tc = time.perf_counter()
+        F.orc_execute(_p(u), _p(out), p, direction, 1)
+        per.append(time.perf_counter() - tc)
+        reps += 1
+    el = time.perf_counter() - t0
+    F.orc_free_fmm(p)
+    return reps / el, reps, el, min(per), sum(per) / len(per)
+
+
 def num_threads():
     return lib().orc_num_threads()
 
