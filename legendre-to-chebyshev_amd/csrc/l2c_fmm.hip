@@ -1722,13 +1722,16 @@ static size_t dense_shared_bytes(int NPP) {
 // (2 x line tiles, row tiles), the parity in the lowest bit of blockIdx.x:
 // the long row tiles of both parities are dispatched first, and consecutive
 // work-groups share a row tile's K rows.
-constexpr int kGemmRT = 64, kGemmLT = 32, kGemmKC = 64, kGemmNT = 256;
+// LDS row stride 33 doubles: the staging writes (lanes along c) and the
+// result reads step 66 dwords, two banks apart, while each 16-lane row of an
+// MFMA operand read stays contiguous
+constexpr int kGemmRT = 64, kGemmLT = 32, kGemmKC = 64, kGemmNT = 256, kGemmLTP = 33;
 template <bool CONTIG>
 __global__ void __launch_bounds__(kGemmNT) __attribute__((amdgpu_waves_per_eu(3)))
     k_lines_gemm(const double *__restrict__ in, double *__restrict__ out, LineMap lm,
                  size_t nlines, int N, int NPP, const double *__restrict__ K) {
   extern __shared__ double sm[];
-  constexpr int LTP = kDenseLTP, KC = kGemmKC, LT = kGemmLT;
+  constexpr int LTP = kGemmLTP, KC = kGemmKC, LT = kGemmLT;
   const int tid = threadIdx.x, lane = tid & 63;
   const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
   const int rt = blockIdx.y, p = blockIdx.x & 1;
@@ -1810,7 +1813,7 @@ __global__ void __launch_bounds__(kGemmNT) __attribute__((amdgpu_waves_per_eu(3)
 }
 
 size_t gemm_shared_bytes() {
-  return 8 * (size_t)kGemmKC * kDenseLTP + kGemmLT * (sizeof(size_t) + sizeof(int));
+  return 8 * (size_t)kGemmKC * kGemmLTP + kGemmLT * (sizeof(size_t) + sizeof(int));
 }
 
 // Batched lines through a direct plan: dense MFMA products when the plan has
