@@ -35,6 +35,9 @@ struct BatchClass {
   int method = 0;        // 0 direct, 1 FMM
   size_t s = 0, L = 0;   // FMM leaf size and depth
   double ms = -1;        // tuner time of the chosen method
+  int alt_method = -1;   // the tuner's runner-up (-1: none) and its time
+  size_t alt_s = 0;
+  double alt_ms = -1;
   double work = 0;       // estimated cost (orders classes over streams)
   int stream = 0;        // stream index the class runs on
   fmm_plan_s *fp = nullptr;
@@ -167,6 +170,69 @@ static std::vector<std::pair<int, size_t>> candidates(int k) {
   return v;
 }
 
+static int launch_batch(l2c_batch_plan_s *bp, const double *d_in, double *d_out, size_t dir,
+                        hipStream_t origin);
+
+// The whole batch, eager, on the staging buffers (min of 3 after one warm-up),
+// milliseconds; < 0 on failure.
+static double time_batch(l2c_batch_plan_s *bp, size_t dir) {
+  hipStream_t st = bp->st[0];
+  hipEvent_t a, b;
+  if (hipEventCreate(&a) != hipSuccess) return -1;
+  if (hipEventCreate(&b) != hipSuccess) {
+    (void)hipEventDestroy(a);
+    return -1;
+  }
+  double best = -1;
+  for (int r = 0; r < 4; r++) {
+    (void)hipEventRecord(a, st);
+    if (launch_batch(bp, bp->d_in, bp->d_out, dir, st)) {
+      best = -1;
+      break;
+    }
+    (void)hipEventRecord(b, st);
+    if (hipEventSynchronize(b) != hipSuccess) {
+      best = -1;
+      break;
+    }
+    float ms = 0;
+    (void)hipEventElapsedTime(&ms, a, b);
+    if (r > 0 && (best < 0 || ms < best)) best = ms;
+  }
+  (void)hipEventDestroy(a);
+  (void)hipEventDestroy(b);
+  return best;
+}
+
+// Classes share the GPU on four streams, so a method that wins alone (a
+// dense product keeping the matrix cores busy) can lose beside the others
+// (latency-bound FMM work overlaps better).  For every class whose runner-up
+// came within 25 % alone, time the whole batch with each and keep the faster.
+static int refine_close_calls(l2c_batch_plan_s *bp, size_t dir) {
+  for (auto &c : bp->cls) {
+    if (c.alt_method < 0 || c.alt_ms < 0 || c.alt_ms > 1.25 * c.ms) continue;
+    const double t_cur = time_batch(bp, dir);
+    const int m0 = c.method;
+    const size_t s0 = c.s;
+    destroy_class_plans(c);
+    if (make_candidate(bp, c, c.alt_method, c.alt_s)) {
+      clear_error();
+      if (make_candidate(bp, c, m0, s0)) return -1;
+      continue;
+    }
+    const double t_alt = time_batch(bp, dir);
+    if (t_cur >= 0 && t_alt >= 0 && t_alt < t_cur) {
+      std::swap(c.ms, c.alt_ms);
+      c.alt_method = m0;
+      c.alt_s = s0;
+      continue;
+    }
+    destroy_class_plans(c);
+    if (make_candidate(bp, c, m0, s0)) return -1;
+  }
+  return 0;
+}
+
 // The untuned rule: direct while 2^k <= 4 maxs (maxs = 64: the reference's
 // own switch to the direct path, src/C/leg2cheb.c:860-866), else FMM s = 64.
 static std::pair<int, size_t> default_method(int k) {
@@ -275,8 +341,8 @@ static l2c_batch_plan_s *create_batch_impl(const size_t *n, size_t count, size_t
       }
       continue;
     }
-    double best = -1;
-    std::pair<int, size_t> pick{-1, 0};
+    double best = -1, second = -1;
+    std::pair<int, size_t> pick{-1, 0}, alt{-1, 0};
     for (const auto &m : candidates(c.k)) {
       if (make_candidate(bp, c, m.first, m.second)) {
         clear_error();
@@ -284,11 +350,20 @@ static l2c_batch_plan_s *create_batch_impl(const size_t *n, size_t count, size_t
       }
       const double ms = time_class(bp, c, tdir);
       destroy_class_plans(c);
-      if (ms >= 0 && (best < 0 || ms < best)) {
+      if (ms < 0) continue;
+      if (best < 0 || ms < best) {
+        second = best;
+        alt = pick;
         best = ms;
         pick = m;
+      } else if (second < 0 || ms < second) {
+        second = ms;
+        alt = m;
       }
     }
+    c.alt_method = alt.first;
+    c.alt_s = alt.second;
+    c.alt_ms = second;
     if (pick.first < 0 || make_candidate(bp, c, pick.first, pick.second)) {
       if (pick.first < 0) set_error("l2c_create_batch: no method ran for a size class");
       destroy_batch(bp);
@@ -323,6 +398,10 @@ static l2c_batch_plan_s *create_batch_impl(const size_t *n, size_t count, size_t
   std::vector<BatchClass> sorted;
   for (size_t i : order) sorted.push_back(std::move(bp->cls[i]));
   bp->cls = std::move(sorted);
+  if (tune && refine_close_calls(bp, tdir)) {
+    destroy_batch(bp);
+    return nullptr;
+  }
   return bp;
 }
 
