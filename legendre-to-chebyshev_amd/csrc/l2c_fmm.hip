@@ -1752,19 +1752,37 @@ __global__ void __launch_bounds__(kGemmNT) __attribute__((amdgpu_waves_per_eu(3)
   const double *Bp = sm + (lane >> 4) * LTP + (lane & 15);
   const double *Kp = K + ((size_t)(p * NR + R) * nkt) * 64 + lane;
   constexpr int QS = KC * LT / kGemmNT;  // staged inputs per thread per chunk
-  // chunk c0's inputs of this thread (line idx / KC, column c0 + idx % KC)
-  // and this wave's fragments, into registers
+  static_assert(KC == 64 && kGemmRT == 64 && kGemmNT == 256, "thread q-th line is 4 q + wave");
+  // this thread's lines in every chunk and in the results: 4 q + wave (q <
+  // QS), column c0 + lane (row 64 rt + lane); their starts and lengths are
+  // wave-uniform and kept in scalar registers
+  const double *src[QS];
+  double *dst[QS];
+  int len[QS];
+#pragma unroll
+  for (int q = 0; q < QS; q++) {
+    const size_t o = toff[4 * q + wave];
+    const size_t ou = (size_t)(unsigned)__builtin_amdgcn_readfirstlane((int)(unsigned)o) |
+                      ((size_t)(unsigned)__builtin_amdgcn_readfirstlane((int)(o >> 32)) << 32);
+    len[q] = __builtin_amdgcn_readfirstlane(tlen[4 * q + wave]);
+    src[q] = in + ou + p * es;
+    dst[q] = out + ou + p * es;
+  }
+  // chunk c0's inputs of this thread and this wave's fragments, into registers
   double e[QS], an[KC / 4];
   auto fetch = [&](int c0) {
     const int ks0 = c0 == kGemmRT * rt ? 4 * wave : 0, nks = min(KC, NPP - c0) >> 2;
+    if (active && ks0 == 0 && nks == KC / 4) {
 #pragma unroll
-    for (int ks = 0; ks < KC / 4; ks++)
-      an[ks] = active && ks >= ks0 && ks < nks ? Kp[((c0 >> 2) + ks) * 64] : 0.0;
+      for (int ks = 0; ks < KC / 4; ks++) an[ks] = Kp[((c0 >> 2) + ks) * 64];
+    } else {
 #pragma unroll
-    for (int q = 0; q < QS; q++) {
-      const int idx = q * kGemmNT + tid, l = idx / KC, j = 2 * (c0 + idx % KC) + p;
-      e[q] = j < tlen[l] ? in[toff[l] + (size_t)j * es] : 0.0;
+      for (int ks = 0; ks < KC / 4; ks++)
+        an[ks] = active && ks >= ks0 && ks < nks ? Kp[((c0 >> 2) + ks) * 64] : 0.0;
     }
+    const int c = c0 + lane;
+#pragma unroll
+    for (int q = 0; q < QS; q++) e[q] = 2 * c + p < len[q] ? src[q][(size_t)(2 * c) * es] : 0.0;
   };
   fetch(kGemmRT * rt);
   for (int c0 = kGemmRT * rt; c0 < NPP; c0 += KC) {
@@ -1774,16 +1792,20 @@ __global__ void __launch_bounds__(kGemmNT) __attribute__((amdgpu_waves_per_eu(3)
     const int nks = min(KC, NPP - c0) >> 2;
     double a[KC / 4];
 #pragma unroll
-    for (int q = 0; q < QS; q++) {
-      const int idx = q * kGemmNT + tid;
-      sm[(idx % KC) * LTP + idx / KC] = e[q];
-    }
+    for (int q = 0; q < QS; q++) sm[lane * LTP + 4 * q + wave] = e[q];
 #pragma unroll
     for (int ks = 0; ks < KC / 4; ks++) a[ks] = an[ks];
     __syncthreads();
     // the next chunk's loads run during this chunk's products
     if (c0 + KC < NPP) fetch(c0 + KC);
-    if (active) {
+    if (active && ks0 == 0 && nks == KC / 4) {
+#pragma unroll
+      for (int ks = 0; ks < KC / 4; ks++) {
+        const double *b = Bp + 4 * ks * LTP;
+        D0 = __builtin_amdgcn_mfma_f64_16x16x4f64(a[ks], b[0], D0, 0, 0, 0);
+        D1 = __builtin_amdgcn_mfma_f64_16x16x4f64(a[ks], b[16], D1, 0, 0, 0);
+      }
+    } else if (active) {
 #pragma unroll
       for (int ks = 0; ks < KC / 4; ks++) {
         if (ks < ks0 || ks >= nks) continue;
@@ -1804,12 +1826,10 @@ __global__ void __launch_bounds__(kGemmNT) __attribute__((amdgpu_waves_per_eu(3)
     }
   }
   __syncthreads();
+  const int r = kGemmRT * rt + lane;
 #pragma unroll
-  for (int q = 0; q < kGemmRT * LT / kGemmNT; q++) {
-    const int idx = q * kGemmNT + tid, l = idx / kGemmRT, rr = idx % kGemmRT;
-    const int r = kGemmRT * rt + rr, i = 2 * r + p;
-    if (r < NPP && i < tlen[l]) out[toff[l] + (size_t)i * es] = sm[rr * LTP + l];
-  }
+  for (int q = 0; q < QS; q++)
+    if (r < NPP && 2 * r + p < len[q]) dst[q][(size_t)(2 * r) * es] = sm[lane * LTP + 4 * q + wave];
 }
 
 size_t gemm_shared_bytes() {
