@@ -323,3 +323,14 @@ def test_planted_mistakes_fail(orc, mistake):
         T = p.T(live=True)
         T[0] = np.roll(T[0], 1, axis=0)
     assert _fmm_err(orc, p, i, 0) > 1e3 * tol(N)
+
+
+def test_ofast_build_times_the_same_transform(orc):
+    """The -Ofast build of the oracle's sources (timed beside the -O3 build
+    for the CPU baseline, never a checker) loads, runs the same plan and
+    reports a positive rate with per-call min <= average."""
+    N = 4096
+    u = np.random.default_rng(3).uniform(-1, 1, N)
+    rate, reps, el, mn, av = orc.ofast_rate(N, 64, 18, 0, u, 0.2, threads=1)
+    assert rate > 0 and reps >= 1 and el > 0
+    assert 0 < mn <= av
