@@ -331,6 +331,10 @@ def test_ofast_build_times_the_same_transform(orc):
     reports a positive rate with per-call min <= average."""
     N = 4096
     u = np.random.default_rng(3).uniform(-1, 1, N)
-    rate, reps, el, mn, av = orc.ofast_rate(N, 64, 18, 0, u, 0.2, threads=1)
+    nt = orc.num_threads()  # both builds share one OpenMP runtime
+    try:
+        rate, reps, el, mn, av = orc.ofast_rate(N, 64, 18, 0, u, 0.2, threads=1)
+    finally:
+        orc.set_num_threads(nt)
     assert rate > 0 and reps >= 1 and el > 0
     assert 0 < mn <= av
