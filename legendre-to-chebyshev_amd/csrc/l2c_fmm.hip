@@ -1717,9 +1717,10 @@ static size_t dense_shared_bytes(int NPP) {
 // memory, L2/MALL-resident) and B = the staged rows, two 16-line column
 // tiles.  The next chunk's inputs and fragments load into registers during
 // the current chunk's products.  The results go out through the same LDS
-// tile.  The two parities are separate work-groups, which halves the longest
-// work-group's chain of chunks (row tile 0 walks all NPP / 64 of them).  Grid
-// (2 x line tiles, row tiles), the parity in the lowest bit of blockIdx.x:
+// tile.  The two parities are separate work-groups: twice as many, each
+// with half the registers and LDS of one holding both, and row tile 0 (which
+// walks all NPP / 64 chunks) splits its chain in two.  Grid (2 x line
+// tiles, row tiles), the parity in the lowest bit of blockIdx.x:
 // the long row tiles of both parities are dispatched first, and consecutive
 // work-groups share a row tile's K rows.
 // LDS row stride 33 doubles: the staging writes (lanes along c) and the
