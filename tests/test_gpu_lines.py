@@ -101,6 +101,25 @@ def test_gemm_lengths(amd, orc, N):
         assert per_line_error(g0, np.ascontiguousarray(o1.T), 0) <= tol(N), d
 
 
+@pytest.mark.parametrize("N", [33, 256])
+def test_dense_many_lines(amd, orc, N):
+    """20,000 lines of N = 33 and 256 in one call: above 16,384 lines the
+    one-tile dense kernel takes them (below, the tiled one); contiguous and
+    strided, both directions, sampled lines against the oracle."""
+    announce(f"dense lines N={N} x 20000")
+    x = np.random.default_rng(N + 11).uniform(-1, 1, (20000, N))
+    op = orc.FMMPlan(N, 64, 18, orc.BOTH)
+    rows = [0, 1, 31, 32, 9999, 19968, 19999]
+    for d in (0, 1):
+        g1 = along_axis(amd, x, 1, d)
+        for r in rows:
+            assert relerr(g1[r], op.execute(np.ascontiguousarray(x[r]), d)) <= tol(N), (d, r)
+        xt = np.ascontiguousarray(x.T)
+        g0 = along_axis(amd, xt, 0, d)
+        for r in rows:
+            assert relerr(g0[:, r], op.execute(np.ascontiguousarray(x[r]), d)) <= tol(N), (d, r)
+
+
 def test_field_round_trip(amd):
     """L2C along the three axes, then C2L back: the input returns
     (src/bin/l2c.c:342, 387 thresholds)."""
