@@ -131,6 +131,37 @@ typedef struct {
 } l2c_plan_info;
 int l2c_get_plan_info(const fmm_plan *plan, l2c_plan_info *info);
 
+/* Plan introspection: copy one of the plan's device tables into host memory
+ * dst (room for `count` doubles) and return its length in doubles; dst = NULL
+ * returns the length alone.  (size_t)-1 with an error text for an unknown
+ * table, one the plan does not hold (an unplanned direction, the FMM tables
+ * of a direct-path plan, P or Q of a tree without composite levels) or too
+ * small a count.  No counterpart in the reference, whose plan struct is
+ * public and whose CLI reads the same tables as fields (src/C/leg2cheb.h:58-78,
+ * src/bin/l2c.c:644-660); tests compare them with the oracle's tables.
+ *   L2C_TAB_LAMBDA  a[k] = Lambda(k), k < N            (src/C/leg2cheb.c:758-795)
+ *   L2C_TAB_AN      the C2L diagonal scale, N           (:796-818)
+ *   L2C_TAB_DN      the C2L band factor, ceil(N/2)      (:796-818)
+ *   L2C_TAB_HEAD    the band head of `direction`: a (L2C) or dn (C2L) for
+ *                   k < 2s, zero-padded to 2s + 16
+ *   L2C_TAB_M2L     the M2L matrices of `direction`, Nc x M x M, level ->
+ *                   block -> submatrix, row-major       (:924-995)
+ *   L2C_TAB_T       2 x s x M [parity][row][k]          (:621-648, :1009-1031)
+ *   L2C_TAB_U       U(0), U(1): child -> parent (modal U(0) = B(0)),
+ *                   row-major M x M                     (:6-123, :1036-1105)
+ *   L2C_TAB_D       D(p) = U(p)^T: parent -> child p
+ *   L2C_TAB_P       P(k, d), k = 1..lu, d < 2^k, at (2^k - 2 + d) M^2:
+ *                   P(1, q) = U(q), P(k, d) = U(d >> (k-1)) P(k-1, d mod 2^(k-1))
+ *   L2C_TAB_Q       Q(k, d), k = 1..ld, d < 2^k, at (2^k - 2 + d) M^2,
+ *                   stored column-major: Q(1, q) = D(q),
+ *                   Q(k, d) = D(d & 1) Q(k-1, d >> 1) */
+enum {
+  L2C_TAB_LAMBDA = 0, L2C_TAB_AN = 1, L2C_TAB_DN = 2, L2C_TAB_HEAD = 3, L2C_TAB_M2L = 4,
+  L2C_TAB_T = 5, L2C_TAB_U = 6, L2C_TAB_D = 7, L2C_TAB_P = 8, L2C_TAB_Q = 9
+};
+size_t l2c_get_plan_table(const fmm_plan *plan, int table, size_t direction, double *dst,
+                          size_t count);
+
 /* Device-resident execute: d_in, d_out are device pointers; runs on `stream`
  * (a hipStream_t, NULL = the plan's own stream) and returns without
  * synchronising.  Same overwrite semantics.  Returns 0 or -1. */

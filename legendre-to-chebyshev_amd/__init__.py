@@ -90,6 +90,8 @@ def lib():
         L.l2c_last_error.restype = C.c_char_p
         L.l2c_get_plan_info.restype = C.c_int
         L.l2c_get_plan_info.argtypes = [vp, C.POINTER(PlanInfo)]
+        L.l2c_get_plan_table.restype = _sz
+        L.l2c_get_plan_table.argtypes = [vp, C.c_int, _sz, vp, _sz]
         L.l2c_execute_device.restype = C.c_int
         L.l2c_execute_device.argtypes = [vp, vp, vp, _sz, _sz, vp]
         L.l2c_execute_lines_device.restype = C.c_int
@@ -280,6 +282,20 @@ class FMMPlan:
         if lib().l2c_get_plan_info(self._plan, C.byref(inf)) != 0:
             raise L2CError(last_error())
         return {k: getattr(inf, k) for k, _ in PlanInfo._fields_}
+
+    TABLES = ("lambda", "an", "dn", "head", "m2l", "T", "U", "D", "P", "Q")
+
+    def table(self, name, direction=L2C):
+        """A copy of one of the plan's device tables (l2c_get_plan_table), as a
+        flat float64 array; see include/leg2cheb_amd.h for layouts."""
+        t = self.TABLES.index(name)
+        n = lib().l2c_get_plan_table(self._plan, t, direction, None, 0)
+        if n == _SIZE_MAX:
+            raise L2CError("l2c_get_plan_table: " + last_error())
+        out = np.empty(n)
+        if lib().l2c_get_plan_table(self._plan, t, direction, out.ctypes.data, n) != n:
+            raise L2CError("l2c_get_plan_table: " + last_error())
+        return out
 
     @property
     def plan(self):

@@ -63,7 +63,9 @@ struct l2c_batch_plan_s {
   const double *g_in = nullptr;
   double *g_out = nullptr;
   size_t g_dir = 99;
-  double *d_in = nullptr, *d_out = nullptr;  // staging for the host-pointer form
+  // device buffers of the tuner (plan time only), then the staging of the
+  // host-pointer form (allocated by its first call)
+  double *d_in = nullptr, *d_out = nullptr;
   double *h_in = nullptr, *h_out = nullptr;
 };
 
@@ -116,7 +118,7 @@ static int make_candidate(l2c_batch_plan_s *bp, BatchClass &c, int method, size_
   c.s = s;
   c.L = 0;
   if (method == 0) {
-    c.dp = make_direct(c.n_max, bp->direction, 0, false);
+    c.dp = make_direct(c.n_max, bp->direction, 0);
     return c.dp ? 0 : -1;
   }
   c.fp = create_fmm_impl(c.n_max, s, bp->M, bp->direction, 0, 0, 0);
@@ -310,12 +312,13 @@ static l2c_batch_plan_s *create_batch_impl(const size_t *n, size_t count, size_t
       hn.push_back((int)n[t]);
     }
   }
+  // the tuner times the candidates on device buffers of the batch's size
+  // (zeros in, scratch out), released once the methods are chosen; the host
+  // staging of l2c_execute_batch is allocated by its first call
   if (hipMalloc(&bp->d_off, count * sizeof(size_t)) != hipSuccess ||
       hipMalloc(&bp->d_n, count * sizeof(int)) != hipSuccess ||
-      hipMalloc(&bp->d_in, tot * sizeof(double)) != hipSuccess ||
-      hipMalloc(&bp->d_out, tot * sizeof(double)) != hipSuccess ||
-      hipHostMalloc((void **)&bp->h_in, tot * sizeof(double)) != hipSuccess ||
-      hipHostMalloc((void **)&bp->h_out, tot * sizeof(double)) != hipSuccess) {
+      (tune && hipMalloc(&bp->d_in, tot * sizeof(double)) != hipSuccess) ||
+      (tune && hipMalloc(&bp->d_out, tot * sizeof(double)) != hipSuccess)) {
     set_error("l2c_create_batch: allocation failed");
     destroy_batch(bp);
     return nullptr;
@@ -325,7 +328,8 @@ static l2c_batch_plan_s *create_batch_impl(const size_t *n, size_t count, size_t
           (destroy_batch(bp), nullptr));
   L2C_HIP(hipMemcpyAsync(bp->d_n, hn.data(), count * sizeof(int), hipMemcpyHostToDevice, s0),
           (destroy_batch(bp), nullptr));
-  L2C_HIP(hipMemsetAsync(bp->d_in, 0, tot * sizeof(double), s0), (destroy_batch(bp), nullptr));
+  if (tune)
+    L2C_HIP(hipMemsetAsync(bp->d_in, 0, tot * sizeof(double), s0), (destroy_batch(bp), nullptr));
   L2C_HIP(hipStreamSynchronize(s0), (destroy_batch(bp), nullptr));
 
   // method per class
@@ -401,6 +405,12 @@ static l2c_batch_plan_s *create_batch_impl(const size_t *n, size_t count, size_t
   if (tune && refine_close_calls(bp, tdir)) {
     destroy_batch(bp);
     return nullptr;
+  }
+  if (tune) {
+    L2C_HIP(hipStreamSynchronize(s0), (destroy_batch(bp), nullptr));
+    (void)hipFree(bp->d_in);
+    (void)hipFree(bp->d_out);
+    bp->d_in = bp->d_out = nullptr;
   }
   return bp;
 }
@@ -492,6 +502,23 @@ int l2c_execute_batch(l2c_batch_plan *bp, const double *in, double *out, size_t 
   }
   const size_t bytes = bp->total * sizeof(double);
   hipStream_t st = bp->st[0];
+  if (!bp->h_in) {  // host staging, on the first call with host arrays
+    double *a = nullptr, *b = nullptr, *c = nullptr, *d = nullptr;
+    if (hipMalloc((void **)&a, bytes) != hipSuccess || hipMalloc((void **)&b, bytes) != hipSuccess ||
+        hipHostMalloc((void **)&c, bytes) != hipSuccess ||
+        hipHostMalloc((void **)&d, bytes) != hipSuccess) {
+      if (a) (void)hipFree(a);
+      if (b) (void)hipFree(b);
+      if (c) (void)hipHostFree(c);
+      if (d) (void)hipHostFree(d);
+      set_error("l2c_execute_batch: host staging allocation failed");
+      return -1;
+    }
+    bp->d_in = a;
+    bp->d_out = b;
+    bp->h_out = d;
+    bp->h_in = c;
+  }
   std::memcpy(bp->h_in, in, bytes);
   L2C_HIP(hipMemcpyAsync(bp->d_in, bp->h_in, bytes, hipMemcpyHostToDevice, st), -1);
   if (execute_batch_impl(bp, bp->d_in, bp->d_out, direction, st)) return -1;

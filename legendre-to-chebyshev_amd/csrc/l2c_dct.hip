@@ -1109,13 +1109,27 @@ int l2c_execute_dct(l2c_dct_plan *p, const double *in, double *out, int kind) {
   }
   const size_t bytes = p->N * p->count * sizeof(double);
   if (!p->st) {
-    if (hipStreamCreate(&p->st) != hipSuccess || hipMalloc((void **)&p->d_in, bytes) != hipSuccess ||
-        hipMalloc((void **)&p->d_out, bytes) != hipSuccess ||
-        hipHostMalloc((void **)&p->h_in, bytes) != hipSuccess ||
-        hipHostMalloc((void **)&p->h_out, bytes) != hipSuccess) {
+    // everything into locals first: the plan takes the stream and buffers
+    // only when all of them exist, so a failed call leaves it unchanged
+    hipStream_t st = nullptr;
+    double *a = nullptr, *b = nullptr, *c = nullptr, *d = nullptr;
+    if (hipStreamCreate(&st) != hipSuccess || hipMalloc((void **)&a, bytes) != hipSuccess ||
+        hipMalloc((void **)&b, bytes) != hipSuccess ||
+        hipHostMalloc((void **)&c, bytes) != hipSuccess ||
+        hipHostMalloc((void **)&d, bytes) != hipSuccess) {
+      if (a) (void)hipFree(a);
+      if (b) (void)hipFree(b);
+      if (c) (void)hipHostFree(c);
+      if (d) (void)hipHostFree(d);
+      if (st) (void)hipStreamDestroy(st);
       set_error("l2c_execute_dct: staging allocation failed");
       return -1;
     }
+    p->d_in = a;
+    p->d_out = b;
+    p->h_in = c;
+    p->h_out = d;
+    p->st = st;
   }
   std::memcpy(p->h_in, in, bytes);
   L2C_HIP(hipMemcpyAsync(p->d_in, p->h_in, bytes, hipMemcpyHostToDevice, p->st), -1);

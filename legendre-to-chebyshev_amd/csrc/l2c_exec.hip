@@ -115,6 +115,51 @@ static bool is_device_ptr(const void *ptr) {
 #endif
 }
 
+// Where the two arrays of a call live: 0 both host, 1 both device, -1 mixed
+// (one host, one device), which no entry point accepts: the host-staged
+// path would read or write device memory from the CPU.
+static int pointer_kind(const void *in, const void *out, const char *fn) {
+  const bool di = is_device_ptr(in), dout = is_device_ptr(out);
+  if (di != dout) {
+    set_error(std::string(fn) + ": input and output must both be device pointers or both host "
+                                "pointers");
+    return -1;
+  }
+  return di ? 1 : 0;
+}
+
+// Whether direction `dir` (0 or 1) was planned by create_fmm's direction
+// argument (L2C, C2L or BOTH); the tables of an unplanned direction do not
+// exist, so every entry point checks before any launch.
+static bool dir_planned(const fmm_plan_s *p, size_t dir) {
+  if (dir > 1) return false;
+  if (p->L == 0) return p->dplan && (p->dplan->direction == BOTH || p->dplan->direction == dir);
+  return p->t.A[dir] != nullptr;
+}
+
+// Host staging buffers (device copy and pinned host copy of N doubles each
+// way), allocated on the first call with host arrays and kept by the plan.
+static int ensure_staging(size_t N, double **din, double **dout, double **hin, double **hout) {
+  if (*din && *dout && *hin && *hout) return 0;
+  double *a = nullptr, *b = nullptr, *c = nullptr, *d = nullptr;
+  if (hipMalloc((void **)&a, N * sizeof(double)) != hipSuccess ||
+      hipMalloc((void **)&b, N * sizeof(double)) != hipSuccess ||
+      hipHostMalloc((void **)&c, N * sizeof(double)) != hipSuccess ||
+      hipHostMalloc((void **)&d, N * sizeof(double)) != hipSuccess) {
+    if (a) (void)hipFree(a);
+    if (b) (void)hipFree(b);
+    if (c) (void)hipHostFree(c);
+    if (d) (void)hipHostFree(d);
+    set_error("host staging allocation failed");
+    return -1;
+  }
+  *din = a;
+  *dout = b;
+  *hin = c;
+  *hout = d;
+  return 0;
+}
+
 }  // namespace l2c
 
 using namespace l2c;
@@ -136,27 +181,30 @@ size_t execute(const double *input_array, double *output_array, fmm_plan *p, siz
     set_error("execute: stride must be >= 1");
     return (size_t)-1;
   }
-  const bool planned = p->L == 0 ? (p->dplan->direction == BOTH || p->dplan->direction == direction)
-                                 : p->t.A[direction] != nullptr;
-  if (!planned) {
+  if (!dir_planned(p, direction)) {
     set_error("execute: direction was not planned");
     return (size_t)-1;
   }
   const size_t N = p->N;
   hipStream_t st = p->stream;
-  const bool dev = is_device_ptr(input_array) && is_device_ptr(output_array);
+  const int kind = pointer_kind(input_array, output_array, "execute");
+  if (kind < 0) return (size_t)-1;
   int rc;
-  if (dev) {
+  if (kind == 1) {
     rc = p->L == 0 ? run_direct(p->dplan, input_array, output_array, direction, stride, st,
                                 p->precompute)
                    : run_fmm(p, input_array, output_array, direction, stride, st, 0);
     if (rc) return (size_t)-1;
     L2C_HIP(hipStreamSynchronize(st), (size_t)-1);
   } else {
-    double *hin = p->L == 0 ? p->dplan->h_in : p->h_in;
-    double *hout = p->L == 0 ? p->dplan->h_out : p->h_out;
-    double *din = p->L == 0 ? p->dplan->d_in : p->d_in;
-    double *dout = p->L == 0 ? p->dplan->d_out : p->d_out;
+    direct_plan_s *dq = p->dplan;
+    if (p->L == 0 ? ensure_staging(N, &dq->d_in, &dq->d_out, &dq->h_in, &dq->h_out)
+                  : ensure_staging(N, &p->d_in, &p->d_out, &p->h_in, &p->h_out))
+      return (size_t)-1;
+    double *hin = p->L == 0 ? dq->h_in : p->h_in;
+    double *hout = p->L == 0 ? dq->h_out : p->h_out;
+    double *din = p->L == 0 ? dq->d_in : p->d_in;
+    double *dout = p->L == 0 ? dq->d_out : p->d_out;
     for (size_t i = 0; i < N; i++) hin[i] = input_array[i * stride];
     L2C_HIP(hipMemcpyAsync(din, hin, N * sizeof(double), hipMemcpyHostToDevice, st), (size_t)-1);
     rc = p->L == 0 ? run_direct(p->dplan, din, dout, direction, 1, st, p->precompute)
@@ -177,10 +225,13 @@ size_t directM(const double *input_array, double *output_array, fmm_plan *p, con
   }
   const size_t N = p->N;
   hipStream_t st = p->stream;
-  if (is_device_ptr(input_array) && is_device_ptr(output_array)) {
+  const int kind = pointer_kind(input_array, output_array, "directM");
+  if (kind < 0) return (size_t)-1;
+  if (kind == 1) {
     if (run_fmm(p, input_array, output_array, L2C, strides, st, 1)) return (size_t)-1;
     L2C_HIP(hipStreamSynchronize(st), (size_t)-1);
   } else {
+    if (ensure_staging(N, &p->d_in, &p->d_out, &p->h_in, &p->h_out)) return (size_t)-1;
     for (size_t i = 0; i < N; i++) p->h_in[i] = input_array[i * strides];
     L2C_HIP(hipMemcpyAsync(p->d_in, p->h_in, N * 8, hipMemcpyHostToDevice, st), (size_t)-1);
     if (run_fmm(p, p->d_in, p->d_out, L2C, 1, st, 1)) return (size_t)-1;
@@ -205,17 +256,20 @@ size_t direct(const double *input_array, double *output_array, direct_plan *dp, 
     set_error("direct: invalid argument");
     return (size_t)-1;
   }
-  if (direction == C2L && !dp->d_an) {
-    set_error("direct: C2L was not planned");
+  if (dp->direction != BOTH && dp->direction != direction) {
+    set_error("direct: direction was not planned");
     return (size_t)-1;
   }
   const size_t N = dp->N;
   hipStream_t st = dp->stream;
-  if (is_device_ptr(input_array) && is_device_ptr(output_array)) {
+  const int kind = pointer_kind(input_array, output_array, "direct");
+  if (kind < 0) return (size_t)-1;
+  if (kind == 1) {
     if (run_direct(dp, input_array, output_array, direction, stride, st, dp->precompute))
       return (size_t)-1;
     L2C_HIP(hipStreamSynchronize(st), (size_t)-1);
   } else {
+    if (ensure_staging(N, &dp->d_in, &dp->d_out, &dp->h_in, &dp->h_out)) return (size_t)-1;
     for (size_t i = 0; i < N; i++) dp->h_in[i] = input_array[i * stride];
     L2C_HIP(hipMemcpyAsync(dp->d_in, dp->h_in, N * 8, hipMemcpyHostToDevice, st), (size_t)-1);
     if (run_direct(dp, dp->d_in, dp->d_out, direction, 1, st, dp->precompute)) return (size_t)-1;
@@ -233,12 +287,12 @@ int l2c_execute_device(fmm_plan *p, const double *d_in, double *d_out, size_t di
     set_error("l2c_execute_device: invalid argument");
     return -1;
   }
-  hipStream_t st = stream ? (hipStream_t)stream : p->stream;
-  if (p->L == 0) return run_direct(p->dplan, d_in, d_out, direction, stride, st, p->precompute);
-  if (!p->t.A[direction]) {
+  if (!dir_planned(p, direction)) {
     set_error("l2c_execute_device: direction was not planned");
     return -1;
   }
+  hipStream_t st = stream ? (hipStream_t)stream : p->stream;
+  if (p->L == 0) return run_direct(p->dplan, d_in, d_out, direction, stride, st, p->precompute);
   return run_fmm(p, d_in, d_out, direction, stride, st, 0);
 }
 
@@ -250,7 +304,7 @@ int l2c_execute_lines_device(fmm_plan *p, const double *d_in, double *d_out, siz
     set_error("l2c_execute_lines_device: invalid argument");
     return -1;
   }
-  if (p->L > 0 && !p->t.A[direction]) {
+  if (!dir_planned(p, direction)) {
     set_error("l2c_execute_lines_device: direction was not planned");
     return -1;
   }
@@ -339,8 +393,14 @@ size_t execute2D(const double *input_array, double *output_array, fmm_plan_2d *q
   }
   const size_t N0 = q->N0, N1 = q->N1, n = N0 * N1;
   fmm_plan_s *pp = q->p0 ? q->p0 : q->p1;
+  if ((q->p0 && !dir_planned(q->p0, direction)) || (q->p1 && !dir_planned(q->p1, direction))) {
+    set_error("execute2D: direction was not planned");
+    return (size_t)-1;
+  }
   hipStream_t st = pp->stream;
-  const bool dev = is_device_ptr(input_array) && is_device_ptr(output_array);
+  const int kind = pointer_kind(input_array, output_array, "execute2D");
+  if (kind < 0) return (size_t)-1;
+  const bool dev = kind == 1;
   const double *din = dev ? input_array : q->d_in;
   double *dout = dev ? output_array : q->d_out;
   if (!dev) L2C_HIP(hipMemcpy(q->d_in, input_array, n * 8, hipMemcpyHostToDevice), (size_t)-1);

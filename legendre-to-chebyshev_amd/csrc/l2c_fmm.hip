@@ -1490,16 +1490,20 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) k
 }
 
 // ---------------------------------------------------------------------------
-// direct O(N^2) sums, one row per thread (src/C/leg2cheb.c:152-234)
+// direct O(N^2) sums, one row per thread (src/C/leg2cheb.c:152-234).  Grid
+// (row blocks, ly, lz): line = blockIdx.z * ly + blockIdx.y, so any number of
+// lines runs in one launch (ly <= 65535, the grid's y limit).
 __global__ void __launch_bounds__(256) k_direct(const double *__restrict__ in,
                                                 double *__restrict__ out, LineMap lm, int N,
                                                 int dir, const double *__restrict__ a,
                                                 const double *__restrict__ an,
-                                                const double *__restrict__ dn) {
+                                                const double *__restrict__ dn, size_t nlines) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  N = line_len(lm, blockIdx.y, N);  // this line's length (tables cover the plan's N)
+  const size_t line = (size_t)blockIdx.z * gridDim.y + blockIdx.y;
+  if (line >= nlines) return;
+  N = line_len(lm, line, N);  // this line's length (tables cover the plan's N)
   if (i >= N) return;
-  const size_t off = line_off(lm, blockIdx.y);
+  const size_t off = line_off(lm, line);
   const double *u = in + off;
   const size_t es = lm.es;
   double acc = 0;
@@ -1838,6 +1842,16 @@ size_t gemm_shared_bytes() {
   return 8 * (size_t)kGemmKC * kGemmLTP + kGemmLT * (sizeof(size_t) + sizeof(int));
 }
 
+// k_direct over nlines lines of map lm (one launch for any number of lines)
+static void launch_direct(const direct_plan_s *dp, const double *d_in, double *d_out, size_t dir,
+                          const LineMap &lm, size_t nlines, hipStream_t st) {
+  const int N = (int)dp->N;
+  const unsigned gy = (unsigned)std::min<size_t>(nlines, dp->lines_y);
+  const unsigned gz = (unsigned)((nlines + gy - 1) / gy);
+  k_direct<<<dim3((N + 255) / 256, gy, gz), 256, 0, st>>>(d_in, d_out, lm, N, (int)dir, dp->d_a,
+                                                          dp->d_an, dp->d_dn, nlines);
+}
+
 // Batched lines through a direct plan: dense MFMA products when the plan has
 // them and the batch is large enough, else one row per thread.
 static int launch_direct_lines(const direct_plan_s *dp, const double *d_in, double *d_out,
@@ -1866,8 +1880,7 @@ static int launch_direct_lines(const direct_plan_s *dp, const double *d_in, doub
       k_lines_dense<false><<<dim3((unsigned)nb), kDenseNT, dense_shared_bytes(dp->NPP), st>>>(
           d_in, d_out, lm, nlines, N, dp->NPP, dp->d_K[dir]);
   } else {
-    k_direct<<<dim3((N + 255) / 256, (unsigned)nlines), 256, 0, st>>>(d_in, d_out, lm, N, (int)dir,
-                                                                      dp->d_a, dp->d_an, dp->d_dn);
+    launch_direct(dp, d_in, d_out, dir, lm, nlines, st);
   }
   L2C_HIP(hipGetLastError(), -1);
   return 0;
@@ -1876,18 +1889,19 @@ static int launch_direct_lines(const direct_plan_s *dp, const double *d_in, doub
 // ---------------------------------------------------------------------------
 // host orchestration
 
-// Workspace for a batch of lines: grown on demand, owned by the plan.  Calls
-// may have run on a caller's stream as well as the plan's own, so the device
-// is drained before the old workspace is released.
+// Workspace for a batch of lines: grown on demand (at least doubling), owned
+// by the plan.  Earlier calls may still be running on a caller's stream, so
+// the store being replaced is retired, not freed: the plan releases it when
+// it is itself freed, and nothing waits here on work of other streams.
 static int ensure_ws(fmm_plan_s *p, size_t nlines, double **w) {
   const size_t need = std::max<size_t>(nlines, 1) * ws_doubles(p->geo);
   if (p->wl_cap < need) {
-    L2C_HIP(hipDeviceSynchronize(), -1);
-    if (p->d_wl) (void)hipFree(p->d_wl);
-    p->d_wl = nullptr;
-    p->wl_cap = 0;
-    L2C_HIP(hipMalloc(&p->d_wl, need * sizeof(double)), -1);
-    p->wl_cap = need;
+    const size_t cap = std::max(need, 2 * p->wl_cap);
+    double *nw = nullptr;
+    L2C_HIP(hipMalloc(&nw, cap * sizeof(double)), -1);
+    if (p->d_wl) p->wl_retired.push_back(p->d_wl);
+    p->d_wl = nw;
+    p->wl_cap = cap;
   }
   *w = p->d_wl;
   return 0;
@@ -1994,10 +2008,8 @@ int run_direct(const direct_plan_s *dp, const double *d_in, double *d_out, size_
   (void)precompute;
   prof_step(dp->prof);
   LineMap lm{1, 0, 0, stride};
-  const int N = (int)dp->N;
   prof_begin(dp->prof, K_DIRECT, st);
-  k_direct<<<dim3((N + 255) / 256, 1), 256, 0, st>>>(d_in, d_out, lm, N, (int)dir, dp->d_a,
-                                                     dp->d_an, dp->d_dn);
+  launch_direct(dp, d_in, d_out, dir, lm, 1, st);
   prof_end(dp->prof, st);
   L2C_HIP(hipGetLastError(), -1);
   return 0;
@@ -2036,25 +2048,13 @@ int run_lines(fmm_plan_s *p, const double *d_in, double *d_out, size_t dir, size
   prof_step(p->prof);
   LineMap lm{n0 ? n0 : 1, ls0, ls1, es};
   if (p->L == 0) {
+    // dense MFMA products when the plan has them (N <= 4096) and there are
+    // at least four lines, else one row per thread; either way one launch
     const direct_plan_s *dp = p->dplan;
-    const int N = (int)dp->N;
-    if (dp->d_K[dir] && nlines >= 4) {
-      prof_begin(dp->prof, K_DIRECT, st);
-      const int rc = launch_direct_lines(dp, d_in, d_out, dir, lm, nlines, st);
-      prof_end(dp->prof, st);
-      return rc;
-    }
-    // lines go in grid.y: at most 65535 per launch
-    for (size_t l0 = 0; l0 < nlines; l0 += 65535) {
-      const size_t nl = std::min<size_t>(65535, nlines - l0);
-      lm.l0 = l0;
-      prof_begin(dp->prof, K_DIRECT, st);
-      k_direct<<<dim3((N + 255) / 256, (unsigned)nl), 256, 0, st>>>(
-          d_in, d_out, lm, N, (int)dir, dp->d_a, dp->d_an, dp->d_dn);
-      prof_end(dp->prof, st);
-      L2C_HIP(hipGetLastError(), -1);
-    }
-    return 0;
+    prof_begin(dp->prof, K_DIRECT, st);
+    const int rc = launch_direct_lines(dp, d_in, d_out, dir, lm, nlines, st);
+    prof_end(dp->prof, st);
+    return rc;
   }
   // lines go in grid.y: at most 65535 per launch, each chunk reusing the
   // front of the plan's line workspace

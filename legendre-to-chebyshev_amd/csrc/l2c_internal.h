@@ -178,6 +178,9 @@ struct direct_plan_s {
   // 2 x NPP x NPP, NPP = ceil(N/2) rounded up to 16
   double *d_K[2] = {nullptr, nullptr};
   int NPP = 0;
+  // lines per grid row of k_direct (grid.y limit 65535; L2C_DIRECT_LINES_Y
+  // sets a smaller value, which the tests use to reach grid.z > 1 cheaply)
+  unsigned lines_y = 65535;
   double *d_in = nullptr, *d_out = nullptr;  // staging (device)
   double *h_in = nullptr, *h_out = nullptr;  // staging (pinned host)
   hipStream_t stream = nullptr;
@@ -195,6 +198,10 @@ struct fmm_plan_s {
   double *d_w = nullptr;           // multipole store: slots x 2 x M
   double *d_wl = nullptr;          // multipole store for line batches
   size_t wl_cap = 0;               // doubles held by d_wl
+  // stores replaced by a larger one; a launch on another stream may still
+  // use them, so they are released only when the plan is freed
+  std::vector<double *> wl_retired;
+  // host staging, allocated by the first call with host arrays
   double *d_in = nullptr, *d_out = nullptr;  // staging (device, contiguous)
   double *h_in = nullptr, *h_out = nullptr;  // staging (pinned host)
   size_t hbm_bytes = 0;
@@ -232,7 +239,7 @@ size_t up_shared_bytes(const Geo &g);
 size_t ws_doubles(const Geo &g);  // execute workspace per line
 size_t flops_direct(size_t N, size_t dir, size_t precompute);
 // implemented in l2c_plan.hip
-direct_plan_s *make_direct(size_t N, size_t direction, size_t precompute, bool staging);
+direct_plan_s *make_direct(size_t N, size_t direction, size_t precompute);
 fmm_plan_s *create_fmm_impl(size_t N, size_t maxs, size_t M, size_t direction, size_t lagrange,
                             size_t precompute, size_t v);
 void destroy_direct(direct_plan_s *dp);

@@ -444,11 +444,15 @@ static int build_lambda_tables(size_t N, bool c2l, double **a, double **an, doub
   return 0;
 }
 
-direct_plan_s *make_direct(size_t N, size_t direction, size_t precompute, bool staging) {
+direct_plan_s *make_direct(size_t N, size_t direction, size_t precompute) {
   direct_plan_s *dp = new direct_plan_s();
   dp->N = N;
   dp->direction = direction;
   dp->precompute = precompute;
+  if (const char *e = getenv("L2C_DIRECT_LINES_Y")) {
+    const long v = atol(e);
+    if (v >= 1 && v <= 65535) dp->lines_y = (unsigned)v;
+  }
   (void)hipGetDevice(&dp->device);
   if (hipStreamCreateWithFlags(&dp->stream, hipStreamNonBlocking) != hipSuccess) {
     set_error("hipStreamCreate failed");
@@ -474,15 +478,6 @@ direct_plan_s *make_direct(size_t N, size_t direction, size_t precompute, bool s
       L2C_HIP(hipGetLastError(), (destroy_direct(dp), nullptr));
     }
     L2C_HIP(hipStreamSynchronize(dp->stream), (destroy_direct(dp), nullptr));
-  }
-  if (staging) {
-    if (dmalloc(&dp->d_in, N, nullptr) || dmalloc(&dp->d_out, N, nullptr) ||
-        hipHostMalloc((void **)&dp->h_in, N * sizeof(double)) != hipSuccess ||
-        hipHostMalloc((void **)&dp->h_out, N * sizeof(double)) != hipSuccess) {
-      set_error("staging allocation failed");
-      destroy_direct(dp);
-      return nullptr;
-    }
   }
   return dp;
 }
@@ -524,6 +519,7 @@ static void destroy_fmm(fmm_plan_s *p) {
   dfree(p->t.head[1]);
   dfree(p->d_w);
   dfree(p->d_wl);
+  for (double *r : p->wl_retired) dfree(r);
   dfree(p->d_in);
   dfree(p->d_out);
   if (p->h_in) (void)hipHostFree(p->h_in);
@@ -599,7 +595,7 @@ fmm_plan_s *create_fmm_impl(size_t N, size_t maxs, size_t M, size_t direction, s
     if (v > 1) printf("Levels < 1. Using only direct method\n");
     p->L = 0;
     p->Nn = N;
-    p->dplan = make_direct(N, direction, precompute, true);
+    p->dplan = make_direct(N, direction, precompute);
     if (!p->dplan) {
       destroy_fmm(p);
       return nullptr;
@@ -818,14 +814,8 @@ fmm_plan_s *create_fmm_impl(size_t N, size_t maxs, size_t M, size_t direction, s
   const size_t slots = slot_base(L);
   (void)slots;
   p->wl_cap = ws_doubles(g);
-  if (dmalloc(&p->d_wl, p->wl_cap, acct) || dmalloc(&p->d_in, N, acct) ||
-      dmalloc(&p->d_out, N, acct)) {
-    destroy_fmm(p);
-    return nullptr;
-  }
-  if (hipHostMalloc((void **)&p->h_in, N * sizeof(double)) != hipSuccess ||
-      hipHostMalloc((void **)&p->h_out, N * sizeof(double)) != hipSuccess) {
-    set_error("hipHostMalloc failed");
+  // (host staging buffers are allocated by the first call with host arrays)
+  if (dmalloc(&p->d_wl, p->wl_cap, acct)) {
     destroy_fmm(p);
     return nullptr;
   }
@@ -860,7 +850,7 @@ direct_plan *create_direct(size_t N, size_t direction, const size_t precompute) 
     set_error("create_direct: need N > 0 and direction in {0,1,2}");
     return nullptr;
   }
-  return make_direct(N, direction, precompute, true);
+  return make_direct(N, direction, precompute);
 }
 
 void free_direct(direct_plan *dplan) { destroy_direct(dplan); }
@@ -885,6 +875,79 @@ int l2c_get_plan_info(const fmm_plan *p, l2c_plan_info *info) {
   info->subtree_up = p->L ? 1 << p->geo.gu : 0;
   info->subtree_down = p->L ? 1 << p->geo.gd : 0;
   return 0;
+}
+
+size_t l2c_get_plan_table(const fmm_plan *p, int table, size_t direction, double *dst,
+                          size_t count) {
+  clear_error();
+  if (!p || direction > 1) {
+    set_error("l2c_get_plan_table: NULL plan or direction not 0/1");
+    return (size_t)-1;
+  }
+  const size_t N = p->N, M = p->M, MM = M * M, s = p->s;
+  const double *src = nullptr;
+  size_t n = 0;
+  const bool fmm = p->L >= 1;
+  const direct_plan_s *dq = p->dplan;
+  switch (table) {
+    case L2C_TAB_LAMBDA:
+      src = fmm ? p->t.a : dq->d_a;
+      n = N;
+      break;
+    case L2C_TAB_AN:
+      src = fmm ? p->t.an : dq->d_an;
+      n = N;
+      break;
+    case L2C_TAB_DN:
+      src = fmm ? p->t.dn : dq->d_dn;
+      n = (N + 1) / 2;
+      break;
+    case L2C_TAB_HEAD:
+      src = fmm ? p->t.head[direction] : nullptr;
+      n = 2 * s + 16;
+      break;
+    case L2C_TAB_M2L:
+      src = fmm ? p->t.A[direction] : nullptr;
+      n = p->Nc * MM;
+      break;
+    case L2C_TAB_T:
+      src = fmm ? p->t.T : nullptr;
+      n = 2 * s * M;
+      break;
+    case L2C_TAB_U:
+      src = fmm ? p->t.U : nullptr;
+      n = 2 * MM;
+      break;
+    case L2C_TAB_D:
+      src = fmm ? p->t.D : nullptr;
+      n = 2 * MM;
+      break;
+    case L2C_TAB_P:
+      src = fmm && p->geo.lu > 0 ? p->t.P : nullptr;
+      n = fmm && p->geo.lu > 0 ? (((size_t)2 << p->geo.lu) - 2) * MM : 0;
+      break;
+    case L2C_TAB_Q:
+      src = fmm && p->geo.ld > 0 ? p->t.Q : nullptr;
+      n = fmm && p->geo.ld > 0 ? (((size_t)2 << p->geo.ld) - 2) * MM : 0;
+      break;
+    default:
+      set_error("l2c_get_plan_table: unknown table");
+      return (size_t)-1;
+  }
+  if (!src) {
+    set_error("l2c_get_plan_table: the plan holds no such table (direction not planned, "
+              "direct-path plan or no composite levels)");
+    return (size_t)-1;
+  }
+  if (!dst) return n;
+  if (count < n) {
+    set_error("l2c_get_plan_table: destination holds fewer doubles than the table");
+    return (size_t)-1;
+  }
+  hipStream_t st = fmm ? p->stream : dq->stream;
+  L2C_HIP(hipStreamSynchronize(st), (size_t)-1);
+  L2C_HIP(hipMemcpy(dst, src, n * sizeof(double), hipMemcpyDeviceToHost), (size_t)-1);
+  return n;
 }
 
 // src/C/leg2cheb.c:125-150

@@ -171,6 +171,19 @@ class FMMPlan:
     def a(self):
         return np.ctypeslib.as_array(self._p.contents.dplan.contents.a, shape=(self.N,)).copy()
 
+    def an(self):
+        return np.ctypeslib.as_array(self._p.contents.dplan.contents.an, shape=(self.N,)).copy()
+
+    def dn(self):
+        return np.ctypeslib.as_array(self._p.contents.dplan.contents.dn,
+                                     shape=((self.N + 1) // 2,)).copy()
+
+    def B(self):
+        """Modal plans: B(0) (M x M); nodal: the two dense child matrices
+        (2 x M x M)."""
+        n = 2 if self._p.contents.lagrange else 1
+        return np.ctypeslib.as_array(self._p.contents.B, shape=(n, self.M, self.M)).copy()
+
     def __del__(self):
         if getattr(self, "_p", None):
             lib().orc_free_fmm(self._p)

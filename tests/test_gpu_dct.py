@@ -85,10 +85,11 @@ def test_more_vectors_than_one_launch(amd, orc, kind):
 
 @pytest.mark.parametrize("kind", [2, 3])
 def test_largest_pass_lengths(amd, kind):
-    """N = 2^23 (the reference's largest timed size: two passes of 2048) and
-    2^25 (three passes of 256, the largest plan) against scipy's pocketfft."""
+    """N = 2^23 (the reference's largest timed size: two passes of 2048),
+    2^24 (three passes of unequal lengths 7/8/8) and 2^25 (three passes of
+    256, the largest plan) against scipy's pocketfft."""
     sf = pytest.importorskip("scipy.fft")
-    for logN in (23, 25):
+    for logN in (23, 24, 25):
         N = 1 << logN
         announce(f"dct kind={kind} N=2^{logN}")
         x = np.random.default_rng(logN + kind).uniform(-1, 1, N)

@@ -139,11 +139,12 @@ def test_field_round_trip(amd):
 @pytest.mark.parametrize("N,maxs", [(512, 64), (600, 256), (200, 64)])
 def test_more_lines_than_one_launch(amd, orc, N, maxs):
     """65,541 lines along the middle axis of a (9363, N, 7) array in one call:
-    the FMM (N = 512) and one-row-per-thread direct (N = 600, maxs = 256)
-    kernels take at most 65,535 lines per launch, so the call runs two, the
-    second starting inside a group of n0 = 7 lines (65,535 = 7 * 9362 + 1);
-    the dense kernel (N = 200) takes them all in one.  The lines either side
-    of the boundary, the first and the last against the oracle."""
+    the FMM kernels (N = 512) take at most 65,535 lines per launch, so the
+    call runs two, the second starting inside a group of n0 = 7 lines
+    (65,535 = 7 * 9362 + 1); the dense products of direct plans (N = 600 on
+    the tiled kernel, N = 200 on the one-tile kernel) take them all in one.
+    The lines either side of the boundary, the first and the last against
+    the oracle."""
     P, Q = 9363, 7
     announce(f"lines across launches N={N} maxs={maxs} nlines={P * Q}")
     x = np.random.default_rng(N).random((P, N, Q)) - 0.5
@@ -155,3 +156,28 @@ def test_more_lines_than_one_launch(amd, orc, N, maxs):
             ref = op.execute(np.ascontiguousarray(x[p, :, q]), d)
             assert relerr(g[p, :, q], ref) <= tol(N), (d, line)
         del g
+
+
+@pytest.mark.parametrize("direction", [0, 1])
+def test_direct_rows_lines_over_grid_z(amd, orc, monkeypatch, direction):
+    """Direct plans above N = 4096 (no dense parity blocks) run their lines
+    on the one-row-per-thread kernel, whose grid holds at most 65,535 lines
+    per grid row and takes the rest in grid.z.  L2C_DIRECT_LINES_Y = 5 at
+    plan creation cuts the rows to five lines, so 13 lines of N = 4100
+    (maxs = 1025: L = 0) span three grid rows; every line against the
+    oracle."""
+    N, P, Q = 4100, 13, 1
+    announce(f"direct rows over grid.z N={N} nlines={P} dir={direction}")
+    monkeypatch.setenv("L2C_DIRECT_LINES_Y", "5")
+    plan = amd.FMMPlan(N, 1025, 18, amd.BOTH)
+    monkeypatch.delenv("L2C_DIRECT_LINES_Y")
+    assert plan.info()["L"] == 0
+    x = np.random.default_rng(41 + direction).uniform(-1, 1, (P, N))
+    din, dout = amd.DeviceBuffer(x.size), amd.DeviceBuffer(x.size)
+    din.upload(x.ravel())
+    plan.execute_lines_device(din.ptr, dout.ptr, direction, P, P, N, 0, 1)
+    plan.synchronize()
+    g = dout.download().reshape(P, N)
+    op = orc.FMMPlan(N, 1025, 18, orc.BOTH)
+    for line in range(P):
+        assert relerr(g[line], op.execute(np.ascontiguousarray(x[line]), direction)) <= tol(N), line

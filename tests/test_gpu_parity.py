@@ -24,9 +24,12 @@ CASES = [(257, 64), (300, 32), (512, 64), (1000, 32), (1025, 64), (4096, 64), (4
 
 @pytest.mark.parametrize("N,maxs", CASES)
 @pytest.mark.parametrize("direction", [0, 1])
-def test_fmm_matches_oracle(amd, orc, N, maxs, direction):
-    announce(f"fmm N={N} maxs={maxs} dir={direction}")
-    u = amd.coefficients(N, seed=1000 + N, decay=0.0 if N % 2 else 1.5)
+@pytest.mark.parametrize("decay", [0.0, 1.5])
+def test_fmm_matches_oracle(amd, orc, N, maxs, direction, decay):
+    """Both input families of the benchmark at every size: U(-1,1) and
+    U(-1,1) (n+1)^-1.5."""
+    announce(f"fmm N={N} maxs={maxs} dir={direction} decay={decay}")
+    u = amd.coefficients(N, seed=1000 + N, decay=decay)
     g = amd.FMMPlan(N, maxs, 18, direction).execute(u, direction)
     o = orc.FMMPlan(N, maxs, 18, direction).execute(u, direction)
     e = relerr(g, o)
@@ -204,3 +207,61 @@ def test_large(amd, orc, N):
                                              np.random.default_rng(N).integers(0, N, 64)]))
             ref = orc.direct_dd_rows(u, rows, 0)
             assert float(np.max(np.abs(g[rows] - ref))) / float(np.max(np.abs(o))) <= tol(N)
+
+
+def _fails(rc, amd, text):
+    """rc is the error return and the error text names the cause."""
+    return rc in (-1, (1 << 64) - 1) and text in amd.last_error()
+
+
+@pytest.mark.parametrize("N", [200, 4096])
+def test_unplanned_direction_rejected(amd, N):
+    """Plans made for L2C alone (a direct-path plan at N = 200 and an FMM
+    plan at N = 4096) refuse C2L at every entry point before any launch:
+    execute, l2c_execute_device, l2c_execute_lines_device, and execute2D
+    on 2-D plans along each axis (the reference dereferences NULL here,
+    src/C/leg2cheb.c:1309-1312)."""
+    announce(f"unplanned direction N={N}")
+    L = amd.lib()
+    p = amd.FMMPlan(N, 64, 18, amd.L2C)
+    d_in, d_out = amd.DeviceBuffer(4 * N), amd.DeviceBuffer(4 * N)
+    d_in.upload(np.zeros(4 * N))
+    u, out = np.ones(N), np.zeros(N)
+    assert _fails(L.execute(u.ctypes.data, out.ctypes.data, p.plan, 1, 1), amd, "not planned")
+    assert _fails(L.l2c_execute_device(p.plan, d_in.ptr, d_out.ptr, 1, 1, None), amd, "not planned")
+    assert _fails(L.l2c_execute_lines_device(p.plan, d_in.ptr, d_out.ptr, 1, 4, 4, N, 0, 1, None),
+                  amd, "not planned")
+    # the planned direction still runs
+    assert L.l2c_execute_lines_device(p.plan, d_in.ptr, d_out.ptr, 0, 4, 4, N, 0, 1, None) == 0
+    p.synchronize()
+    for axis in (0, 1, -1):
+        q = L.create_fmm_2d(N, 8, axis, 64, 18, amd.L2C, 0, 0, 0)
+        assert q
+        x, y = np.ones(8 * N), np.zeros(8 * N)
+        assert _fails(L.execute2D(x.ctypes.data, y.ctypes.data, q, 1), amd, "not planned")
+        assert L.execute2D(x.ctypes.data, y.ctypes.data, q, 0) != (1 << 64) - 1
+        L.free_fmm_2d(q)
+
+
+@pytest.mark.parametrize("N", [200, 4096])
+def test_mixed_host_device_pointers_rejected(amd, N):
+    """A host input with a device output (or the reverse) is an error at
+    execute, direct and directM: the host-staged path would touch device
+    memory from the CPU."""
+    announce(f"mixed pointers N={N}")
+    L = amd.lib()
+    p = amd.FMMPlan(N, 64, 18, amd.BOTH)
+    dbuf = amd.DeviceBuffer(N)
+    dbuf.upload(np.zeros(N))
+    h = np.zeros(N)
+    for a, b in ((h.ctypes.data, dbuf.ptr), (dbuf.ptr, h.ctypes.data)):
+        assert _fails(L.execute(a, b, p.plan, 0, 1), amd, "both be device pointers or both host")
+        if N > 256:
+            assert _fails(L.directM(a, b, p.plan, 1), amd, "both be device pointers or both host")
+    dp = L.create_direct(N, amd.BOTH, 0)
+    assert dp
+    assert _fails(L.direct(h.ctypes.data, dbuf.ptr, dp, 0, 1), amd, "both be device pointers or both host")
+    L.free_direct(dp)
+    # matching kinds still work
+    u = amd.coefficients(N, seed=3)
+    assert relerr(p.execute(u, 0), amd.FMMPlan(N, 64, 18, 0).execute(u, 0)) == 0.0

@@ -338,3 +338,22 @@ def test_ofast_build_times_the_same_transform(orc):
         orc.set_num_threads(nt)
     assert rate > 0 and reps >= 1 and el > 0
     assert 0 < mn <= av
+
+
+# ------------------------------------------------------------ Table 1
+@pytest.mark.parametrize("N", [256, 512, 1024, 2048, 4096, 8192])
+def test_oracle_table1_envelope(orc, N):
+    """The oracle reproduces the reference's published accuracy (Table 1,
+    FMM_paper.pdf p.17; tests/golden/table1_published.json): maxs = 32,
+    M = 18, modal and nodal, L2C and C2L, U(0,1) inputs, absolute error
+    against the __float128 direct sum within the published error plus
+    four ulps of ||z*||_inf."""
+    from conftest import TABLE1_SLACK_ULPS, table1_error, table1_inputs, table1_published
+    pub = table1_published()[N]
+    u = table1_inputs(N)
+    for d in (0, 1):
+        ref = orc.leg2cheb_q(u) if d == 0 else orc.cheb2leg_q(u)
+        for lag in (0, 1):
+            g = orc.FMMPlan(N, 32, 18, d, lag).execute(u, d)
+            e, ulp = table1_error(g, ref)
+            assert e <= pub[(d, lag)] + TABLE1_SLACK_ULPS * ulp, (d, lag, e, e / ulp)
