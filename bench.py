@@ -443,6 +443,9 @@ def main():
                     help="rows checked against the double-double direct sum")
     ap.add_argument("--prof-every", type=int, default=16,
                     help="time the kernels of every n-th step with HIP events")
+    ap.add_argument("--graph", action="store_true",
+                    help="replay a HIP graph of each step's launches (measured slower than "
+                         "launching them directly)")
     ap.add_argument("--compare-maxs", type=int, default=32,
                     help="also time this maxs (the published figure's leaf size); 0 = skip")
     ap.add_argument("--pcie-steps", type=int, default=20,
@@ -563,6 +566,9 @@ def main():
         log(f"[bench] calibration copy {args.copy_mb} MiB")
         copy_calibration(amd, args.copy_mb)
 
+    # --graph: steps replay a HIP graph of the launches (l2c_plan_graph)
+    if args.graph:
+        plan.graph(True)
     log(f"[bench] warmup {args.warmup}")
     for _ in range(args.warmup):
         plan.execute_device(din.ptr, dout.ptr, d)
@@ -786,6 +792,7 @@ def main():
                 + (f", times (n+1)^-{args.decay:g})" if args.decay else ")"),
         "config": {"workload": workload, "N": N, "maxs": maxs, "s": info["s"], "L": info["L"],
                    "M": M, "direction": "L2C" if d == 0 else "C2L", "form": "plain-fp64",
+                   "launch": "hip-graph replay" if args.graph else "direct",
                    "inputs_resident": "HBM", "plan_seconds": round(plan_s, 4)},
         "max_rel_err": {"vs_oracle_fmm": err_fmm, "vs_dd_direct_rows": err_direct,
                         "rows_checked": int(rows.size), "tol": tol},

@@ -168,6 +168,17 @@ size_t l2c_get_plan_table(const fmm_plan *plan, int table, size_t direction, dou
 int l2c_execute_device(fmm_plan *plan, const double *d_in, double *d_out,
                        size_t direction, size_t stride, void *stream);
 
+/* HIP-graph replay of l2c_execute_device (no counterpart in the reference).
+ * With enable = 1 the plan captures its execute launch sequence into a HIP
+ * graph on the first l2c_execute_device call for a given (d_in, d_out,
+ * direction, stride) and replays it on later calls with the same arguments
+ * (the launches are identical; replay saves host launch work).  Up to eight
+ * argument sets are kept; further ones launch directly, as do the calls that
+ * l2c_profile times (their kernels run between the timing events).
+ * enable = 0 releases the graphs (after synchronising the plan's stream).
+ * Returns 0 or -1. */
+int l2c_plan_graph(fmm_plan *plan, int enable);
+
 /* Many lines of length N through one plan in one call (one launch sequence
  * per 65,535 lines; any nlines).  Line l (l < nlines) has first element at
  * (l / n0) * line_stride1 + (l % n0) * line_stride0 and its elements at that

@@ -94,6 +94,8 @@ def lib():
         L.l2c_get_plan_table.argtypes = [vp, C.c_int, _sz, vp, _sz]
         L.l2c_execute_device.restype = C.c_int
         L.l2c_execute_device.argtypes = [vp, vp, vp, _sz, _sz, vp]
+        L.l2c_plan_graph.restype = C.c_int
+        L.l2c_plan_graph.argtypes = [vp, C.c_int]
         L.l2c_execute_lines_device.restype = C.c_int
         L.l2c_execute_lines_device.argtypes = [vp, vp, vp, _sz, _sz, _sz, _sz, _sz, _sz, vp]
         L.l2c_set_precision.restype = C.c_int
@@ -251,6 +253,11 @@ class FMMPlan:
         if rc != 0:
             raise L2CError("l2c_execute_lines_device: " + last_error())
 
+    def graph(self, enable=True):
+        """Replay execute_device launches from cached HIP graphs (l2c_plan_graph)."""
+        if lib().l2c_plan_graph(self._plan, 1 if enable else 0) != 0:
+            raise L2CError(last_error())
+
     def set_precision(self, form):
         if lib().l2c_set_precision(self._plan, form) != 0:
             raise L2CError(last_error())
@@ -260,7 +267,7 @@ class FMMPlan:
             raise L2CError(last_error())
 
     # execute kernels by profile slot
-    KERNELS = ("k_up", "k_m2l", "k_down", "k_direct", "reserved4", "reserved5", "reserved6",
+    KERNELS = ("k_up", "k_m2l", "k_down", "k_direct", "k_far", "k_band", "reserved6",
                "reserved7")
 
     def profile(self, enable=True):

@@ -115,6 +115,58 @@ static bool is_device_ptr(const void *ptr) {
 #endif
 }
 
+// One execute through a cached HIP graph of its launch sequence: captured on
+// the plan's stream by the first call with this key, replayed on st after.
+// A call that l2c_profile times runs its launches directly between its
+// events (timing events stay out of the graphs), as does a call whose key
+// finds the cache full.  Returns 0 or -1.
+static int run_fmm_graph(fmm_plan_s *p, const double *d_in, double *d_out, size_t dir,
+                         size_t stride, hipStream_t st) {
+  prof_step(p->prof);
+  if ((p->prof && p->prof->on && p->prof->cur) ||
+      (int)p->graphs.size() >= fmm_plan_s::kMaxGraphs) {
+    bool cached = false;
+    for (const auto &g : p->graphs)
+      cached = cached || (g.in == d_in && g.out == d_out && g.dir == dir && g.stride == stride &&
+                          g.ws == p->d_wl && g.precision == p->precision);
+    if (!cached || (p->prof && p->prof->on && p->prof->cur))
+      return run_fmm_launch(p, d_in, d_out, dir, stride, st, 0);
+  }
+  for (const auto &g : p->graphs)
+    if (g.in == d_in && g.out == d_out && g.dir == dir && g.stride == stride && g.ws == p->d_wl &&
+        g.precision == p->precision) {
+      L2C_HIP(hipGraphLaunch(g.exec, st), -1);
+      return 0;
+    }
+  hipStream_t cap = p->stream;
+  L2C_HIP(hipStreamBeginCapture(cap, hipStreamCaptureModeThreadLocal), -1);
+  const int rc = run_fmm_launch(p, d_in, d_out, dir, stride, cap, 0);
+  hipGraph_t graph = nullptr;
+  const hipError_t e = hipStreamEndCapture(cap, &graph);
+  if (rc || e != hipSuccess || !graph) {
+    if (graph) (void)hipGraphDestroy(graph);
+    if (!rc) set_error(std::string("execute graph capture failed: ") + hipGetErrorString(e));
+    return -1;
+  }
+  hipGraphExec_t exec = nullptr;
+  const hipError_t ei = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
+  (void)hipGraphDestroy(graph);
+  if (ei != hipSuccess) {
+    set_error(std::string("hipGraphInstantiate: ") + hipGetErrorString(ei));
+    return -1;
+  }
+  p->graphs.push_back({d_in, d_out, dir, stride, p->d_wl, p->precision, exec});
+  L2C_HIP(hipGraphLaunch(exec, st), -1);
+  return 0;
+}
+
+void drop_graphs(fmm_plan_s *p) {
+  if (p->graphs.empty()) return;
+  if (p->stream) (void)hipStreamSynchronize(p->stream);
+  for (auto &g : p->graphs) (void)hipGraphExecDestroy(g.exec);
+  p->graphs.clear();
+}
+
 // Where the two arrays of a call live: 0 both host, 1 both device, -1 mixed
 // (one host, one device), which no entry point accepts: the host-staged
 // path would read or write device memory from the CPU.
@@ -293,6 +345,7 @@ int l2c_execute_device(fmm_plan *p, const double *d_in, double *d_out, size_t di
   }
   hipStream_t st = stream ? (hipStream_t)stream : p->stream;
   if (p->L == 0) return run_direct(p->dplan, d_in, d_out, direction, stride, st, p->precompute);
+  if (p->graph_on) return run_fmm_graph(p, d_in, d_out, direction, stride, st);
   return run_fmm(p, d_in, d_out, direction, stride, st, 0);
 }
 
@@ -310,6 +363,17 @@ int l2c_execute_lines_device(fmm_plan *p, const double *d_in, double *d_out, siz
   }
   hipStream_t st = stream ? (hipStream_t)stream : p->stream;
   return run_lines(p, d_in, d_out, direction, nlines, n0, ls0, ls1, es, st);
+}
+
+int l2c_plan_graph(fmm_plan *p, int enable) {
+  clear_error();
+  if (!p) {
+    set_error("l2c_plan_graph: NULL plan");
+    return -1;
+  }
+  if (!enable) drop_graphs(p);
+  p->graph_on = enable != 0;
+  return 0;
 }
 
 int l2c_profile(fmm_plan *p, int enable) {

@@ -1128,7 +1128,7 @@ bool down8_eligible(const Geo &g) { return g.s == 16 || g.s == 32 || g.s == 64; 
 // lgkm counter, so the half-trip prefetch is not drained by scalar waits).
 // RAGGED: columns
 // t >= len are masked (the last chunk, where len is not a multiple of 8).
-template <int DIR, int PRE, bool RAGGED>
+template <int DIR, int PRE, bool RAGGED, bool LEAN = false>
 __device__ __forceinline__ void band_seg8(const double *__restrict__ xsg,
                                           const double *__restrict__ pw,
                                           const double *__restrict__ head, int h0, int sh,
@@ -1190,6 +1190,7 @@ __device__ __forceinline__ void band_seg8(const double *__restrict__ xsg,
     TERM(hr[((K) + 2) & 7], pr[((K) + 6) & 7], x_, acc[6]) \
     TERM(hr[((K) + 1) & 7], pr[((K) + 7) & 7], x_, acc[7]) \
     pr[K] = P[J];                                          \
+    if (LEAN) __builtin_amdgcn_sched_barrier(0);           \
   }
   FETCH(XA, PA, HA, 0)
   for (int b = 0; b < ntrip; b++) {
@@ -1479,6 +1480,299 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) k
   for (int e = tid; e < nr; e += NT) {
     const size_t i = row0 + e;
     double v = ob[slot(e)] + ob2[slot(e)];
+    if (DIR == 0) {
+      v *= k2Pi;
+      if (i == 0) v *= 0.5;
+    } else {
+      v *= (double)i + 0.5;
+    }
+    dst[i * lm.es] = v;
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Split downward pass: k_band (the near field) and k_far (L2L and the leaf
+// evaluation), each lean enough that several work-groups share a CU, so one
+// work-group's loads and barriers overlap another's arithmetic.  k_band has
+// no input from the tree: it runs first and writes the raw band sums (the
+// C2L diagonal included) over the output; k_far then adds the far field and
+// applies the final scaling (src/C/leg2cheb.c:1467-1483 and the band scaling
+// of :236-535).  Both use k_down8's work-group partition (2^gd leaf chunks,
+// s/16 waves), band lane layout and LDS windows.
+struct BandLds {
+  int xs, nxs, lw, LW, hd, total;
+};
+__host__ __device__ inline BandLds band_lds(const Geo &g) {
+  const Down8Lds d = down8_lds(g);
+  return {d.xs, d.nxs, d.lw, d.LW, d.hd, d.hd + 2 * g.s + 16};
+}
+size_t band_shared_bytes(const Geo &g) { return 8 * (size_t)band_lds(g).total; }
+
+// Band sums of the work-group's rows, written over the output: raw for
+// k_far, or with the final scaling when scale != 0 (the band alone,
+// directM).  2 s/16 waves: wave w < s/16 runs segment A of row pair w, wave
+// s/16 + w segment B of row pair s/16 - 1 - w (k_down8's split of each row
+// group's columns, one segment per wave), so no wave holds two sets of sums
+// and the kernel fits 128 VGPRs.
+template <int DIR, int PRE>
+__global__ void __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(4))) k_band(
+    const double *__restrict__ in, double *__restrict__ out, LineMap lm, Geo g, int scale,
+    const double *__restrict__ tab, const double *__restrict__ head) {
+  extern __shared__ double sm[];
+  const int s = g.s, N = g.N;
+  const int Nl = line_len(lm, blockIdx.y, N);
+  const int G = 1 << g.gd, sh = 31 - __builtin_clz(s);
+  const int nw = s >> 4, NT = 128 * nw;
+  const int Rr = blockIdx.x;
+  const int tid = threadIdx.x, lane = tid & 63;
+  const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
+  const size_t row0 = (size_t)Rr * G * 2 * s;
+  if (row0 >= (size_t)Nl) return;  // the whole work-group
+  const int nr = (int)std::min((size_t)G * 2 * s, (size_t)Nl - row0);
+  const size_t hc0 = (size_t)Rr * G * s;
+  const size_t off = line_off(lm, blockIdx.y);
+  const double *src = in + off;
+  double *dst = out + off;
+  const BandLds ly = band_lds(g);
+  double *xs = sm + ly.xs, *lw = sm + ly.lw, *hd = sm + ly.hd;
+  double *ob = xs, *ob2 = lw;  // staging after the band (see k_down8)
+  auto slot = [&](int e) { return e + (e >> (sh + 1)); };
+
+  // ---- loads: the x window, the Lambda span, the head (one batch)
+  const int nX = 2 * ((G + 1) * s + 8);
+  constexpr int QX = 5;  // ceil(nX / NT) for G = 16
+  double vx[QX];
+  const int QL = 3 * s + 16, nLs = 2 * G * s + s + 16;
+  constexpr int QLS = 5;
+  double vl[QLS];
+  const int nhd = 2 * s + 16;
+#pragma unroll
+  for (int q = 0; q < QX; q++) vx[q] = src[std::min(row0 + q * NT + tid, (size_t)Nl - 1) * lm.es];
+#pragma unroll
+  for (int q = 0; q < QLS; q++) vl[q] = tab[std::min(2 * hc0 + q * NT + tid, (size_t)N - 1)];
+  const double vh = head[tid < nhd ? tid : 0];
+#pragma unroll
+  for (int q = 0; q < QX; q++) {
+    const int e = q * NT + tid;
+    if (e < nX) {
+      const size_t i = row0 + e;
+      const double u = i < (size_t)Nl ? vx[q] : 0.0;
+      const int h = e >> 1;
+      xs[(e & 1) * ly.nxs + h + (h >> sh)] = (DIR == 1 && i > 0) ? u * (double)i : u;
+    }
+  }
+  for (int e = QX * NT + tid; e < nX; e += NT) {
+    const size_t i = row0 + e;
+    double u = i < (size_t)Nl ? src[i * lm.es] : 0.0;
+    if (DIR == 1 && i > 0) u *= (double)i;
+    const int h = e >> 1;
+    xs[(e & 1) * ly.nxs + h + (h >> sh)] = u;
+  }
+  auto put_lambda = [&](int e, double v) {
+    const int c = e >> (sh + 1), q = e - (c << (sh + 1));
+    if (2 * hc0 + e >= (size_t)N) v = 0.0;
+    if (c < G) lw[c * ly.LW + q] = v;
+    if (c > 0 && q + 2 * s < QL) lw[(c - 1) * ly.LW + q + 2 * s] = v;
+  };
+#pragma unroll
+  for (int q = 0; q < QLS; q++) {
+    const int e = q * NT + tid;
+    if (e < nLs) put_lambda(e, vl[q]);
+  }
+  for (int e = QLS * NT + tid; e < nLs; e += NT)
+    put_lambda(e, tab[std::min(2 * hc0 + e, (size_t)N - 1)]);
+  if (tid < nhd) hd[tid] = vh;
+  lds_sync();
+
+  // ---- the band: lane (chunk gl, parity sg, del), one segment per wave
+  const int gl = lane & 15, sg = (lane >> 4) & 1, del = lane >> 5;
+  const int cnt = (N - sg + 1) / 2;
+  const size_t beta = (size_t)Rr * G + gl;
+  const size_t nLm1 = (size_t)g.nL - 1;
+  const size_t cend = beta < nLm1 ? (beta + 2) * s : (size_t)cnt;
+  const bool ragged = (size_t)(Rr + 1) * G > nLm1;
+  const double *xsg = xs + sg * ly.nxs;
+  const bool segB = wave >= nw;
+  const int p = segB ? 2 * nw - 1 - wave : wave, gam = 2 * p + del, ts = s - 8 * p;
+  const size_t r0 = beta * s + 8 * gam;
+  const bool live = gl < G && r0 < (size_t)cnt;
+  const int len = live ? (int)(cend - r0) : 0;
+  const int h0 = (int)(r0 - hc0);
+  const double *pw = lw + gl * ly.LW + 16 * gam + sg;
+  double acc[8];
+#pragma unroll
+  for (int k = 0; k < 8; k++) acc[k] = 0.0;
+  if (!segB) {
+    if (DIR == 1 && live) {  // C2L diagonal: sqrt(pi) an_i v_i (src/C/leg2cheb.c:416)
+      const double *xr = xsg + h0 + (h0 >> sh);
+#pragma unroll
+      for (int k = 0; k < 8; k++) acc[k] = kSqrtPi * xr[k] * pw[2 * k];
+    }
+    if (ragged)
+      band_seg8<DIR, PRE, true, true>(xsg, pw, hd, h0, sh, 0, std::min(ts, len), len, acc);
+    else
+      band_seg8<DIR, PRE, false, true>(xsg, pw, hd, h0, sh, 0, ts, len, acc);
+  } else {
+    if (ragged)
+      band_seg8<DIR, PRE, true, true>(xsg, pw, hd, h0, sh, ts, len, len, acc);
+    else
+      band_seg8<DIR, PRE, false, true>(xsg, pw, hd, h0, sh, ts, len, len, acc);
+  }
+  lds_sync();  // windows dead: the staging takes their place
+  if (live) {
+    double *dstg = segB ? ob2 : ob;
+#pragma unroll
+    for (int k = 0; k < 8; k++)
+      if (hc0 + h0 + k < (size_t)cnt) dstg[slot(2 * (h0 + k) + sg)] = acc[k];
+  }
+  lds_sync();
+  for (int e = tid; e < nr; e += NT) {
+    const size_t i = row0 + e;
+    double v = ob[slot(e)] + ob2[slot(e)];
+    if (scale) {
+      if (DIR == 0) {
+        v *= k2Pi;
+        if (i == 0) v *= 0.5;
+      } else {
+        v *= (double)i + 0.5;
+      }
+    }
+    dst[i * lm.es] = v;
+  }
+}
+
+// k_far: L2L down the ancestor chain and the subtree (as k_down8), the
+// subtree's local expansions at the leaf nodes on MFMA, added to the band
+// sums k_band left in the output, scaled and stored.
+struct FarLds {
+  int dl, cres, mres, qp, ob, total;
+};
+__host__ __device__ inline FarLds far_lds(const Geo &g) {
+  const int G = 1 << g.gd, s = g.s, M2 = 2 * g.M;
+  FarLds d;
+  d.dl = 0;  // D(0), D(1) with rows padded to M + 1
+  d.cres = d.dl + g.M * M2 + M2;
+  d.mres = d.cres + (2 * G - 1) * M2;
+  d.qp = d.mres + (g.ld + 2 * G - 1) * M2;
+  d.ob = d.qp + std::max(g.ld, 1) * M2;
+  d.total = d.ob + 2 * G * s + G;  // far field, slot(e) = e + e / 2s
+  return d;
+}
+size_t far_shared_bytes(const Geo &g) { return 8 * (size_t)far_lds(g).total; }
+
+template <int DIR, int MT>
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) k_far(
+    double *__restrict__ out, LineMap lm, Geo g, const double *__restrict__ wsall,
+    size_t wsstride, const double *__restrict__ Dg, const double *__restrict__ Qg,
+    const double *__restrict__ Tg) {
+  extern __shared__ double sm[];
+  const int M = MT > 0 ? MT : g.M;
+  const int s = g.s, N = g.N, M2 = 2 * M, MM = M * M;
+  const int Nl = line_len(lm, blockIdx.y, N);
+  const int G = 1 << g.gd, sh = 31 - __builtin_clz(s);
+  const int nw = s >> 4, NT = 64 * nw;
+  const int Rr = blockIdx.x;
+  const int tid = threadIdx.x, lane = tid & 63;
+  const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
+  const size_t row0 = (size_t)Rr * G * 2 * s;
+  if (row0 >= (size_t)Nl) return;
+  const int nr = (int)std::min((size_t)G * 2 * s, (size_t)Nl - row0);
+  const size_t off = line_off(lm, blockIdx.y);
+  double *dst = out + off;
+  const double *m2l = wsall + blockIdx.y * wsstride + ws_layout(g).m2l;
+  const FarLds ly = far_lds(g);
+  double *dl = sm + ly.dl, *cres = sm + ly.cres, *mres = sm + ly.mres, *qp = sm + ly.qp;
+  double *ob = sm + ly.ob;
+  auto slot = [&](int e) { return e + (e >> (sh + 1)); };
+
+  // ---- loads: the tree operands (M2L vectors, the chain row of Q, D), the
+  // band sums to add to (used last), T for the leaf evaluation
+  const int nm = (g.ld + 2 * G - 1) * M2;
+  constexpr int QM = 6;
+  double vm[QM];
+  bool okm[QM];
+  const int nq = g.ld * M;
+  double qr[MT > 0 ? MT : 1];
+  constexpr int QD = MT > 0 ? (2 * MT * MT + 63) / 64 : 1;
+  double vd[QD];
+#pragma unroll
+  for (int q = 0; q < QM; q++) {
+    const int e = q * NT + tid;
+    const size_t a = m2l_elem(g, Rr, e < nm ? e : 0, M2, okm[q]);
+    okm[q] = okm[q] && e < nm;
+    vm[q] = m2l[okm[q] ? a : 0];
+  }
+  if (MT > 0 && nq > 0) {
+    const double *qrow = qrow_ptr(Qg, g, Rr, tid < nq ? tid : 0, M);
+#pragma unroll
+    for (int l = 0; l < (MT > 0 ? MT : 1); l++) qr[l] = qrow[l * M];
+  }
+#pragma unroll
+  for (int q = 0; q < QD; q++) {
+    const int e = q * NT + tid;
+    vd[q] = Dg[e < 2 * MM ? e : 0];
+  }
+  __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+  for (int q = 0; q < QM; q++) {
+    const int e = q * NT + tid;
+    if (e < nm) mres[e] = okm[q] ? vm[q] : 0.0;
+  }
+  for (int e = QM * NT + tid; e < nm; e += NT) {
+    bool ok;
+    const size_t a = m2l_elem(g, Rr, e, M2, ok);
+    mres[e] = ok ? m2l[a] : 0.0;
+  }
+#pragma unroll
+  for (int q = 0; q < QD; q++) {
+    const int e = q * NT + tid;
+    if (MT > 0 && e < 2 * MM) dl[(e / M) * (M + 1) + e % M] = vd[q];
+  }
+  for (int e = (MT > 0 ? QD * NT : 0) + tid; e < 2 * MM; e += NT)
+    dl[(e / M) * (M + 1) + e % M] = Dg[e];
+  lds_sync();
+
+  subtree_locals<MT, true>(g, Rr, tid, NT, mres, qp, cres, qr, nullptr, nullptr, Qg, Dg, dl);
+
+  // ---- far field: F_par^T (G x s) = C_par^T (G x M) T_par^T (M x s) on
+  // v_mfma_f64_16x16x4f64, two tiles (parity, 16 rows) per wave
+  const int ntm = s >> 4, ksteps = (M + 3) >> 2;
+  constexpr int QK = MT > 0 ? (MT + 3) / 4 : 1;
+  const int n = lane & 15;
+#pragma unroll
+  for (int j = 0; j < 2; j++) {
+    const int tile = wave + j * nw, sgf = tile / ntm, rt = tile - sgf * ntm;
+    d4 D = {0, 0, 0, 0};
+    if (MT > 0) {
+      double av[QK], bv[QK];
+#pragma unroll
+      for (int kk = 0; kk < QK; kk++) {
+        const int k = kk * 4 + (lane >> 4);
+        bv[kk] = k < M ? Tg[((size_t)sgf * s + rt * 16 + n) * M + k] : 0.0;
+        av[kk] = (n < G && k < M) ? cres[(G - 1 + n) * M2 + sgf * M + k] : 0.0;
+      }
+#pragma unroll
+      for (int kk = 0; kk < QK; kk++) D = __builtin_amdgcn_mfma_f64_16x16x4f64(av[kk], bv[kk], D, 0, 0, 0);
+    } else {
+      for (int kk = 0; kk < ksteps; kk++) {
+        const int k = kk * 4 + (lane >> 4);
+        const double bv = k < M ? Tg[((size_t)sgf * s + rt * 16 + n) * M + k] : 0.0;
+        const double av = (n < G && k < M) ? cres[(G - 1 + n) * M2 + sgf * M + k] : 0.0;
+        D = __builtin_amdgcn_mfma_f64_16x16x4f64(av, bv, D, 0, 0, 0);
+      }
+    }
+#pragma unroll
+    for (int v = 0; v < 4; v++) {
+      const int leaf = (lane >> 4) + 4 * v, m = rt * 16 + n;
+      const int e = 2 * (leaf * s + m) + sgf;
+      if (leaf < G && e < nr) ob[slot(e)] = D[v];
+    }
+  }
+  lds_sync();
+  // ---- the band sums plus the far field, scaled
+  for (int e = tid; e < nr; e += NT) {
+    const size_t i = row0 + e;
+    double v = dst[i * lm.es] + ob[slot(e)];
     if (DIR == 0) {
       v *= k2Pi;
       if (i == 0) v *= 0.5;
@@ -1931,6 +2225,48 @@ static void launch_down8(fmm_plan_s *p, const double *d_in, double *d_out, size_
   prof_end(p->prof, st);
 }
 
+// The split downward pass on st: k_band (before the tree kernels, which it
+// does not need), then k_up and k_m2l, then k_far.
+template <int MT>
+static void launch_band(fmm_plan_s *p, const double *d_in, double *d_out, size_t dir,
+                        const LineMap &lm, size_t nlines, hipStream_t st, int scale) {
+  const Geo &g = p->geo;
+  dim3 gd((unsigned)((size_t)4 << g.ld), (unsigned)nlines);
+  const size_t shm = band_shared_bytes(g);
+  const int nt = 128 * (g.s >> 4);
+  const double *tab = dir == L2C ? p->t.a : p->t.an;
+  const double *htab = p->t.head[dir];
+  prof_begin(p->prof, K_BAND, st);
+  if (dir == L2C)
+    k_band<0, 0><<<gd, nt, shm, st>>>(d_in, d_out, lm, g, scale, tab, htab);
+  else if (p->precompute)
+    k_band<1, 1><<<gd, nt, shm, st>>>(d_in, d_out, lm, g, scale, tab, htab);
+  else
+    k_band<1, 0><<<gd, nt, shm, st>>>(d_in, d_out, lm, g, scale, tab, htab);
+  prof_end(p->prof, st);
+}
+
+template <int MT>
+static void launch_far(fmm_plan_s *p, double *d_out, size_t dir, const LineMap &lm,
+                       size_t nlines, hipStream_t st, double *ws) {
+  const Geo &g = p->geo;
+  dim3 gd((unsigned)((size_t)4 << g.ld), (unsigned)nlines);
+  const int nt = 64 * (g.s >> 4);
+  prof_begin(p->prof, K_FAR, st);
+  if (dir == L2C)
+    k_far<0, MT><<<gd, nt, far_shared_bytes(g), st>>>(d_out, lm, g, ws, ws_doubles(g), p->t.D,
+                                                      p->t.Q, p->t.T);
+  else
+    k_far<1, MT><<<gd, nt, far_shared_bytes(g), st>>>(d_out, lm, g, ws, ws_doubles(g), p->t.D,
+                                                      p->t.Q, p->t.T);
+  prof_end(p->prof, st);
+}
+
+// Whether this call takes the split downward pass (k_band + k_far).
+static bool use_split(const fmm_plan_s *p, size_t nlines) {
+  return p->geo.fast && (p->down_split == 1 || (p->down_split == 2 && nlines > 1));
+}
+
 // One execute (or a batch of lines): upward pass, M2L, downward pass, all on st.
 template <int MT>
 static int launch_fmm_m(fmm_plan_s *p, const double *d_in, double *d_out, size_t dir,
@@ -1941,6 +2277,11 @@ static int launch_fmm_m(fmm_plan_s *p, const double *d_in, double *d_out, size_t
   if (lm.n && !g.fast) {  // per-line lengths: k_up and k_down8 only
     set_error("lines of uneven length need a plan with leaf size s = 16, 32 or 64");
     return -1;
+  }
+  const bool split = use_split(p, nlines);
+  if (split) {
+    launch_band<MT>(p, d_in, d_out, dir, lm, nlines, st, band_only);
+    if (band_only) return 0;
   }
   if (!band_only) {
     dim3 gu((unsigned)((size_t)4 << g.lu), (unsigned)nlines);
@@ -1963,6 +2304,10 @@ static int launch_fmm_m(fmm_plan_s *p, const double *d_in, double *d_out, size_t
     k_m2l<MT><<<gm, kM2lNT, m2l_shared_bytes(g), st>>>(g, p->t.A[dir], ws, wsstride, bpw,
                                                       nlines, lpb, sv);
     prof_end(p->prof, st);
+  }
+  if (split) {
+    launch_far<MT>(p, d_out, dir, lm, nlines, st, ws);
+    return 0;
   }
   if (g.fast) {
     launch_down8<MT>(p, d_in, d_out, dir, lm, nlines, st, band_only ? K8_BAND : K8_FULL, ws);
@@ -1996,11 +2341,15 @@ static int launch_fmm(fmm_plan_s *p, const double *d_in, double *d_out, size_t d
   L2C_HIP(hipGetLastError(), -1);
   return 0;
 }
+int run_fmm_launch(fmm_plan_s *p, const double *d_in, double *d_out, size_t dir, size_t stride,
+                   hipStream_t st, int band_only) {
+  LineMap lm{1, 0, 0, stride};
+  return launch_fmm(p, d_in, d_out, dir, lm, 1, st, band_only);
+}
 int run_fmm(fmm_plan_s *p, const double *d_in, double *d_out, size_t dir, size_t stride,
             hipStream_t st, int band_only) {
   prof_step(p->prof);
-  LineMap lm{1, 0, 0, stride};
-  return launch_fmm(p, d_in, d_out, dir, lm, 1, st, band_only);
+  return run_fmm_launch(p, d_in, d_out, dir, stride, st, band_only);
 }
 
 int run_direct(const direct_plan_s *dp, const double *d_in, double *d_out, size_t dir,

@@ -150,7 +150,9 @@ struct DevTables {
 };
 
 // Per-launch HIP-event timing of the execute kernels (bench support).
-enum KernelId { K_UP_SUBTREE = 0, K_M2L = 1, K_DOWN_BAND = 2, K_DIRECT = 3, K_COUNT = 8 };
+enum KernelId {
+  K_UP_SUBTREE = 0, K_M2L = 1, K_DOWN_BAND = 2, K_DIRECT = 3, K_FAR = 4, K_BAND = 5, K_COUNT = 8
+};
 struct Prof {
   int on = 0;       // timing period in calls (0: off)
   size_t calls = 0;  // calls since enabled
@@ -192,6 +194,9 @@ struct fmm_plan_s {
   size_t Nn = 0, L = 0, s = 0, Nc = 0, Nb = 0;
   int device = 0;
   int precision = 0;  // 0 plain FP64, 1 split form of the M2L matrices
+  // downward pass of s = 16/32/64 plans: 0 the fused k_down8, 1 k_band +
+  // k_far, 2 k_band + k_far for calls of more than one line
+  int down_split = 0;
   l2c::Geo geo{};
   l2c::DevTables t;
   direct_plan_s *dplan = nullptr;  // L < 1: direct path only
@@ -207,6 +212,20 @@ struct fmm_plan_s {
   size_t hbm_bytes = 0;
   hipStream_t stream = nullptr;
   l2c::Prof *prof = nullptr;
+  // HIP graphs of the execute launch sequence (l2c_plan_graph): one per
+  // (input, output, direction, stride, workspace, precision form), at most
+  // kMaxGraphs; further keys run the launches directly
+  struct Graph {
+    const double *in;
+    double *out;
+    size_t dir, stride;
+    const double *ws;
+    int precision;
+    hipGraphExec_t exec;
+  };
+  static constexpr int kMaxGraphs = 8;
+  bool graph_on = false;
+  std::vector<Graph> graphs;
 };
 
 struct fmm_plan_2d_s {
@@ -225,6 +244,9 @@ constexpr int kGemmMaxN = 4096;
 // implemented in l2c_exec.hip
 int run_fmm(fmm_plan_s *p, const double *d_in, double *d_out, size_t dir, size_t stride,
             hipStream_t st, int band_only);
+// the same without advancing the timing period (the caller did)
+int run_fmm_launch(fmm_plan_s *p, const double *d_in, double *d_out, size_t dir, size_t stride,
+                   hipStream_t st, int band_only);
 int run_direct(const direct_plan_s *dp, const double *d_in, double *d_out, size_t dir,
                size_t stride, hipStream_t st, size_t precompute);
 int run_lines(fmm_plan_s *p, const double *d_in, double *d_out, size_t dir, size_t nlines,
@@ -232,8 +254,11 @@ int run_lines(fmm_plan_s *p, const double *d_in, double *d_out, size_t dir, size
 int run_packed(fmm_plan_s *p, const direct_plan_s *dp, const double *d_in, double *d_out,
                size_t dir, size_t nlines, const size_t *d_off, const int *d_n, hipStream_t st);
 int reserve_lines(fmm_plan_s *p, size_t nlines);
+void drop_graphs(fmm_plan_s *p);  // synchronises the plan's stream first
 size_t flops_fmm(const fmm_plan_s *p, size_t dir);
 size_t down_shared_bytes(const Geo &g);  // of the downward kernel g selects
+size_t band_shared_bytes(const Geo &g);
+size_t far_shared_bytes(const Geo &g);
 bool down8_eligible(const Geo &g);
 size_t up_shared_bytes(const Geo &g);
 size_t ws_doubles(const Geo &g);  // execute workspace per line

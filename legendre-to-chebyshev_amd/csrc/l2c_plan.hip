@@ -501,6 +501,7 @@ static void destroy_fmm(fmm_plan_s *p) {
   if (!p) return;
   // nothing may still read the tables or the workspace
   if (p->stream) (void)hipStreamSynchronize(p->stream);
+  drop_graphs(p);
   dfree(p->t.a);
   dfree(p->t.an);
   dfree(p->t.dn);
@@ -583,6 +584,7 @@ fmm_plan_s *create_fmm_impl(size_t N, size_t maxs, size_t M, size_t direction, s
   p->direction = direction;
   p->lagrange = lagrange;
   p->precompute = precompute;
+  if (const char *e = getenv("L2C_DOWN_SPLIT")) p->down_split = std::max(0, std::min(2, atoi(e)));
   (void)hipGetDevice(&p->device);
   if (hipStreamCreateWithFlags(&p->stream, hipStreamNonBlocking) != hipSuccess) {
     set_error("hipStreamCreate failed");
@@ -1062,6 +1064,7 @@ int l2c_set_precision(fmm_plan *p, int form) {
     set_error("l2c_set_precision: form must be 0 (plain FP64) or 1 (split M2L matrices)");
     return -1;
   }
+  drop_graphs(p);  // their launches carry the previous form's arguments
   if (form == 1 && p->L >= 1) {
     L2C_HIP(hipStreamSynchronize(p->stream), -1);
     for (int d = 0; d < 2; d++)
