@@ -189,6 +189,22 @@ int l2c_execute_lines_device(fmm_plan *plan, const double *d_in, double *d_out,
                              size_t line_stride0, size_t line_stride1,
                              size_t elem_stride, void *stream);
 
+/* l2c_execute_lines_device for callers that cannot allocate (code under
+ * HIP graph capture, tight serving loops, harnesses that forbid hipMalloc).
+ * No counterpart in the reference.  Same arguments, semantics and results;
+ * the FMM workspace (multipoles and M2L vectors) is a fixed 32 MiB device
+ * array instead of the plan's line workspace, so the call allocates nothing
+ * (and times nothing).  Lines run as many per launch as the array holds,
+ * launches in order on `stream`.  Returns -1 with an error text when one
+ * line of the plan needs more workspace than the array holds (at M = 18 and
+ * maxs = 64: N above 2^21).  Every call on a GPU shares that GPU's array:
+ * calls that overlap on one GPU (another stream, another thread, another
+ * plan) race and must be ordered by the caller. */
+int l2c_execute_lines_device_noalloc(fmm_plan *plan, const double *d_in, double *d_out,
+                                     size_t direction, size_t nlines, size_t n0,
+                                     size_t line_stride0, size_t line_stride1,
+                                     size_t elem_stride, void *stream);
+
 /* Batches of independent transforms of uneven length (no counterpart in the
  * reference, whose callers loop over execute(); src/C/leg2cheb.c:1272-1293
  * is the same loop for the lines of a 2-D array).

@@ -94,6 +94,8 @@ def lib():
         L.l2c_get_plan_table.argtypes = [vp, C.c_int, _sz, vp, _sz]
         L.l2c_execute_device.restype = C.c_int
         L.l2c_execute_device.argtypes = [vp, vp, vp, _sz, _sz, vp]
+        L.l2c_execute_lines_device_noalloc.restype = C.c_int
+        L.l2c_execute_lines_device_noalloc.argtypes = [vp, vp, vp, _sz, _sz, _sz, _sz, _sz, _sz, vp]
         L.l2c_plan_graph.restype = C.c_int
         L.l2c_plan_graph.argtypes = [vp, C.c_int]
         L.l2c_execute_lines_device.restype = C.c_int

@@ -27,6 +27,7 @@
 
 #include <algorithm>
 #include <cmath>
+#include <mutex>
 
 #include "l2c_internal.h"
 
@@ -2415,4 +2416,98 @@ int run_lines(fmm_plan_s *p, const double *d_in, double *d_out, size_t dir, size
   return 0;
 }
 
+// ---------------------------------------------------------------------------
+// Workspace of l2c_execute_lines_device_noalloc: a fixed device array, one
+// copy per GPU, held until the process ends.  Host code never names it in a
+// launch or copy: it passes the per-device address from hipGetSymbolAddress.
+constexpr size_t kScratchDoubles = (size_t)4 << 20;  // 32 MiB
+static __device__ __align__(256) double scratch[kScratchDoubles];
+
+constexpr int kMaxDevices = 64;
+static double *g_scratch[kMaxDevices] = {};
+static bool g_scratch_done[kMaxDevices] = {};
+static std::mutex g_scratch_mu;
+
+// The scratch address on the current device, looked up once per device (with
+// that device current, as the calling thread has it).  nullptr on failure.
+static double *scratch_ptr() {
+  int dev = 0;
+  if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= kMaxDevices) {
+    set_error("l2c_execute_lines_device_noalloc: no current device");
+    return nullptr;
+  }
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  if (!g_scratch_done[dev]) {
+    void *ptr = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !ptr) {
+      set_error(std::string("hipGetSymbolAddress(scratch): ") + hipGetErrorString(e));
+      return nullptr;
+    }
+    g_scratch[dev] = (double *)ptr;
+    g_scratch_done[dev] = true;
+  }
+  return g_scratch[dev];
+}
+
+// Lines through an FMM plan with the fixed scratch as the workspace: as many
+// lines per launch as the scratch holds (and at most 65,535), launches in
+// order on st, each reusing the scratch front.  Nothing here allocates, and
+// nothing in the scratch needs clearing (every element a launch reads was
+// written earlier in the same launch sequence).
+static int run_lines_scratch(fmm_plan_s *p, double *scr, const double *d_in, double *d_out,
+                             size_t dir, size_t nlines, size_t n0, size_t ls0, size_t ls1,
+                             size_t es, hipStream_t st) {
+  LineMap lm{n0 ? n0 : 1, ls0, ls1, es};
+  if (p->L == 0) return launch_direct_lines(p->dplan, d_in, d_out, dir, lm, nlines, st);
+  const size_t wsd = ws_doubles(p->geo);
+  const size_t per = std::min<size_t>(65535, kScratchDoubles / wsd);
+  if (per == 0) {
+    set_error("l2c_execute_lines_device_noalloc: one line of this plan needs " +
+              std::to_string(wsd * sizeof(double)) + " bytes of workspace, more than the " +
+              std::to_string(kScratchDoubles * sizeof(double)) + "-byte scratch");
+    return -1;
+  }
+  for (size_t l0 = 0; l0 < nlines; l0 += per) {
+    lm.l0 = l0;
+    const size_t nl = std::min(per, nlines - l0);
+    const int rc = p->geo.M == 18 ? launch_fmm_m<18>(p, d_in, d_out, dir, lm, nl, st, 0, scr)
+                                  : launch_fmm_m<0>(p, d_in, d_out, dir, lm, nl, st, 0, scr);
+    if (rc) return rc;
+    L2C_HIP(hipGetLastError(), -1);
+  }
+  return 0;
+}
+
 }  // namespace l2c
+
+using namespace l2c;
+
+extern "C" {
+
+int l2c_execute_lines_device_noalloc(fmm_plan *p, const double *d_in, double *d_out,
+                                     size_t direction, size_t nlines, size_t n0, size_t ls0,
+                                     size_t ls1, size_t es, void *stream) {
+  clear_error();
+  if (!p || !d_in || !d_out || direction > 1 || es == 0 || nlines == 0) {
+    set_error("l2c_execute_lines_device_noalloc: invalid argument");
+    return -1;
+  }
+  const bool planned = p->L == 0 ? (p->dplan->direction == BOTH || p->dplan->direction == direction)
+                                 : p->t.A[direction] != nullptr;
+  if (!planned) {
+    set_error("l2c_execute_lines_device_noalloc: direction was not planned");
+    return -1;
+  }
+  double *scr = scratch_ptr();
+  if (!scr) return -1;
+  // no timing events here (their pool would grow by allocation)
+  const int cur = p->prof ? p->prof->cur : 0;
+  if (p->prof) p->prof->cur = 0;
+  const int rc = run_lines_scratch(p, scr, d_in, d_out, direction, nlines, n0, ls0, ls1, es,
+                                   stream ? (hipStream_t)stream : p->stream);
+  if (p->prof) p->prof->cur = cur;
+  return rc;
+}
+
+}  // extern "C"
