@@ -235,12 +235,14 @@ def bench_batch(amd, orc, args):
     return out
 
 
-def bench_field(amd, args):
+def bench_field(amd, args, steps=None):
     """A 256^3 FP64 field c_ijk = U(-1,1)/((i+1)(j+1)(k+1)) resident in HBM:
     L2C along axes 2, 1, 0 (65,536 strided lines of N = 256 per axis, one
     batched call each), then C2L back along 0, 1, 2; 64 vectors of N = 2^18
     (L2C) run at the same time on the batch plan's own streams.  One step is
-    the six field passes plus the 64 vectors.  Prints one JSON line."""
+    the six field passes plus the 64 vectors.  The six passes are also timed
+    alone, and their dense line kernel with HIP events for its roofline.
+    Returns the figures as a dict."""
     from oracle import orc
     n = args.field_n
     shape = (n, n, n)
@@ -296,25 +298,53 @@ def bench_field(amd, args):
             err_vec = max(err_vec, float(np.max(np.abs(gv[t * nvec:(t + 1) * nvec] - ref)) / np.max(np.abs(ref))))
     for _ in range(args.warmup // 10 + 2):
         step()
-    steps = max(5, args.steps // 20)
+    steps = steps or max(5, args.steps // 20)
     t0 = time.perf_counter()
     for _ in range(steps):
         step()
     el = (time.perf_counter() - t0) / steps
+    # the six passes alone, then once more with HIP events around each pass's
+    # kernel (the dense line product, one launch per pass)
+    nv_saved, nv = nv, 0
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        step()
+    el_alone = (time.perf_counter() - t0) / steps
+    fp.profile(1)
+    for _ in range(max(2, steps // 2)):
+        step()
+    prof = fp.profile_read()
+    fp.profile(False)
+    nv = nv_saved
+    kname, (kms, kcnt) = next(iter(prof.items()))
+    pass_us = 1e3 * kms / kcnt
+    # algorithmic work of one pass: every line read and written once; the
+    # dense product per line and parity is the upper triangle of K_p at
+    # 16 x 16 tile granularity (NPP = 128: 36 tiles of 16 x 16 x 2 flops x 16)
+    npp = 128 if n == 256 else ((n + 1) // 2 + 15) // 16 * 16
+    tiles = (npp // 16) * (npp // 16 + 1) // 2
+    pass_bytes = 16 * n * n * n
+    pass_flops = n * n * 2 * tiles * 256 * 2
     lines = 6 * n * n
-    out = {"metric": "field lines/s: 256^3 FP64 field, L2C along 3 axes then C2L back, "
-                     "with 64 x 2^18 L2C transforms mixed in",
-           "value": round(lines / el, 1), "unit": "lines/s", "n_gpus": 1, "steps": steps,
-           "ms_per_step": round(1e3 * el, 4), "higher_is_better": True, "dtype": "f64",
-           "data": "synthetic (SplitMix64)", "transforms_per_s": round((lines + nv) / el, 1),
+    out = {"lines_per_s": round(lines / el, 1), "transforms_per_s": round((lines + nv) / el, 1),
+           "ms_per_step": round(1e3 * el, 4), "steps": steps,
+           "alone": {"lines_per_s": round(lines / el_alone, 1),
+                     "ms_per_6_passes": round(1e3 * el_alone, 4)},
+           "dense_kernel": {"kernel": "k_lines_dense", "avg_pass_us": round(pass_us, 2),
+                            "algorithmic_bytes": pass_bytes,
+                            "achieved_GBps": round(pass_bytes / (pass_us * 1e-6) / 1e9, 1),
+                            "hbm_frac": round(pass_bytes / (pass_us * 1e-6) / 1e9 / HBM_PEAK_GBS, 4),
+                            "flops": pass_flops,
+                            "achieved_TFLOPs": round(pass_flops / (pass_us * 1e-6) / 1e12, 2),
+                            "fp64_mfma_frac": round(pass_flops / (pass_us * 1e-6) / 78.6e12, 4)},
            "config": {"workload": f"field{n}_x6_plus_{nv}x2^18", "field": list(shape),
                       "vectors": nv, "vector_n": nvec, "M": args.M},
            "max_rel_err": {"field_lines_vs_oracle": err_lines, "vectors_vs_oracle": err_vec},
-           "round_trip_max_abs_err": rt}
-    print(json.dumps(out), flush=True)
-    if not (err_lines <= 1e-14 * math.log2(n) and err_vec <= 1e-14 * 18 and rt < 1e-10):
-        log("[bench] FAILED: field error above tolerance")
-        sys.exit(1)
+           "round_trip_max_abs_err": rt,
+           "tol": {"field_lines": 1e-14 * math.log2(n), "vectors": 1e-14 * 18, "round_trip": 1e-10}}
+    out["ok"] = bool(err_lines <= 1e-14 * math.log2(n) and err_vec <= 1e-14 * 18 and rt < 1e-10)
+    log(f"[bench] field: {out}")
+    return out
 
 
 FFTW_REDFT10_2_20 = {"value": round(1 / 6.263459e-03, 2), "unit": "transforms/s",
@@ -411,6 +441,35 @@ def bench_dct(amd, orc, args, steps):
     return out
 
 
+def host_cpu():
+    """The host CPU the baseline runs on: model name, physical cores and
+    logical CPUs from /proc/cpuinfo, and the CPUs this process may use."""
+    model, cores, logical = None, set(), 0
+    try:
+        with open("/proc/cpuinfo") as f:
+            phys = core = None
+            for line in f:
+                k, _, v = line.partition(":")
+                k, v = k.strip(), v.strip()
+                if k == "processor":
+                    logical += 1
+                elif k == "model name" and model is None:
+                    model = v
+                elif k == "physical id":
+                    phys = v
+                elif k == "core id":
+                    core = v
+                    cores.add((phys, core))
+    except OSError:
+        pass
+    try:
+        avail = len(os.sched_getaffinity(0))
+    except AttributeError:
+        avail = os.cpu_count()
+    return {"model": model, "physical_cores": len(cores) or None, "logical_cpus": logical or None,
+            "cpus_available_to_process": avail}
+
+
 def relerr_(x, ref):
     return float(np.max(np.abs(x - ref)) / np.max(np.abs(ref)))
 
@@ -427,6 +486,9 @@ def main():
     ap.add_argument("--dct-steps", type=int, default=200,
                     help="timed DCT-II calls in the default line (0 = skip)")
     ap.add_argument("--field-n", type=int, default=256)
+    ap.add_argument("--field-steps", type=int, default=5,
+                    help="timed steps of the 256^3 field (with its 2^18 vectors) in the default "
+                         "line (0 = skip)")
     ap.add_argument("--field-vectors", type=int, default=64)
     ap.add_argument("--steps", type=int, default=400)
     ap.add_argument("--warmup", type=int, default=40)
@@ -441,8 +503,11 @@ def main():
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--check-rows", type=int, default=512,
                     help="rows checked against the double-double direct sum")
-    ap.add_argument("--prof-every", type=int, default=16,
-                    help="time the kernels of every n-th step with HIP events")
+    ap.add_argument("--prof-every", type=int, default=1,
+                    help="time the kernels with HIP events in a separate pass after the timed "
+                         "steps (0 = skip)")
+    ap.add_argument("--settle", type=float, default=0.25,
+                    help="seconds of untimed transforms before the warmup steps")
     ap.add_argument("--graph", action="store_true",
                     help="replay a HIP graph of each step's launches (measured slower than "
                          "launching them directly)")
@@ -488,7 +553,21 @@ def main():
 
     amd = load_pkg()
     if args.workload == "field":
-        bench_field(amd, args)
+        f = bench_field(amd, args)
+        line = {"metric": "field lines/s: 256^3 FP64 field, L2C along 3 axes then C2L back, "
+                          "with 64 x 2^18 L2C transforms mixed in",
+                "value": f["lines_per_s"], "unit": "lines/s", "n_gpus": 1, "steps": f["steps"],
+                "ms_per_step": f["ms_per_step"], "higher_is_better": True, "dtype": "f64",
+                "data": "synthetic (SplitMix64)", "config": f["config"],
+                "roofline": {"bound": "hbm", "kernel": "k_lines_dense",
+                             "achieved": f["dense_kernel"]["achieved_GBps"], "peak": HBM_PEAK_GBS,
+                             "unit": "GB/s", "frac": f["dense_kernel"]["hbm_frac"],
+                             "traffic": None},
+                "field": f}
+        print(json.dumps(line), flush=True)
+        if not f["ok"]:
+            log("[bench] FAILED: field error above tolerance")
+            sys.exit(1)
         return
     if args.workload == "dct":
         sys.path.insert(0, ROOT)
@@ -569,26 +648,43 @@ def main():
     # --graph: steps replay a HIP graph of the launches (l2c_plan_graph)
     if args.graph:
         plan.graph(True)
+    # bring the GPU to its steady clocks before the warmup steps: untimed
+    # transforms for --settle seconds (a few warmup steps alone leave the
+    # first timed steps slower than the rest)
+    t0 = time.perf_counter()
+    nsettle = 0
+    while time.perf_counter() - t0 < args.settle:
+        for _ in range(50):
+            plan.execute_device(din.ptr, dout.ptr, d)
+        plan.synchronize()
+        nsettle += 50
     log(f"[bench] warmup {args.warmup}")
     for _ in range(args.warmup):
         plan.execute_device(din.ptr, dout.ptr, d)
     plan.synchronize()
 
     log(f"[bench] timed {args.steps} steps")
-    # kernel durations from HIP events around the kernels of every
-    # --prof-every-th step (events between kernels cost GPU time themselves)
-    plan.profile(args.prof_every)
     plan.synchronize()
     t0 = time.perf_counter()
     for _ in range(args.steps):
         plan.execute_device(din.ptr, dout.ptr, d)
     plan.synchronize()
     elapsed = time.perf_counter() - t0
-    prof = plan.profile_read()
-    plan.profile(False)
     value = args.steps / elapsed
     ms_per_step = 1e3 * elapsed / args.steps
-    kernels = {k: {"avg_us": 1e3 * ms / max(c, 1), "launches": c} for k, (ms, c) in prof.items()}
+    # kernel durations: a separate pass of the same steps with HIP events on
+    # the launch stream around every kernel (kept out of the timed loop,
+    # whose throughput the events would lower)
+    prof_steps = max(20, args.steps // 4) if args.prof_every > 0 else 0
+    kernels = {}
+    if prof_steps:
+        plan.profile(1)
+        for _ in range(prof_steps):
+            plan.execute_device(din.ptr, dout.ptr, d)
+        plan.synchronize()
+        prof = plan.profile_read()
+        plan.profile(False)
+        kernels = {k: {"avg_us": 1e3 * ms / max(c, 1), "launches": c} for k, (ms, c) in prof.items()}
     log(f"[bench] {value:.1f} transforms/s, {ms_per_step * 1e3:.2f} us/step; kernels {kernels}")
 
     # correctness against the oracle
@@ -732,11 +828,32 @@ def main():
         e2 = float(np.max(np.abs(g2 - o2)) / np.max(np.abs(o2)))
         cmp_line = {"maxs": args.compare_maxs, "s": p2.info()["s"], "value": round(v2, 2),
                     "vs_published": round(v2 / PUBLISHED["value"], 1), "max_rel_err_vs_oracle": e2}
+        if not args.no_cpu:
+            # the oracle at the same leaf size on the host cores, all threads
+            # and one core (the published figure is one core)
+            op2 = orc.FMMPlan(N, args.compare_maxs, M, d)
+            out2 = np.zeros(N)
+            op2.execute(u, d, out=out2)
+            cpu2 = {}
+            nt = orc.num_threads()
+            for label, thr, secs in (("all_threads", nt, args.cpu_seconds / 4),
+                                     ("one_core", 1, args.cpu_seconds / 4)):
+                orc.set_num_threads(thr)
+                reps2, t0 = 0, time.perf_counter()
+                while time.perf_counter() - t0 < secs:
+                    op2.execute(u, d, out=out2)
+                    reps2 += 1
+                cpu2[label] = {"value": round(reps2 / (time.perf_counter() - t0), 3), "cores": thr,
+                               "sample": f"{reps2} repeats in {time.perf_counter() - t0:.1f} s"}
+            orc.set_num_threads(nt)
+            cmp_line["cpu_baseline"] = dict(cpu2, kind="port", unit="transforms/s")
+            del op2
         log(f"[bench] maxs={args.compare_maxs}: {cmp_line}")
         del p2
 
     batched = bench_batch(amd, orc, args) if args.batch > 0 else None
     dct = bench_dct(amd, orc, args, args.dct_steps) if args.dct_steps > 0 else None
+    field = bench_field(amd, args, steps=5) if args.field_steps > 0 else None
 
     cpu = None
     if not args.no_cpu:
@@ -758,7 +875,7 @@ def main():
         cpu = {"value": round(reps / cpu_t, 3), "unit": "transforms/s", "cores": nt, "kind": "port",
                "sample": f"oracle FMM execute, N={N} L2C maxs={maxs} M={M}, {reps} repeats "
                          f"in {cpu_t:.1f} s, OpenMP {nt} threads, own loops (no BLAS)",
-               "min_s": min(per), "avg_s": sum(per) / len(per)}
+               "min_s": min(per), "avg_s": sum(per) / len(per), "host": host_cpu()}
         log(f"[bench] cpu baseline {cpu}")
         # the same restatement on one core (the reference's published figures
         # are single-core)
@@ -793,6 +910,7 @@ def main():
         "config": {"workload": workload, "N": N, "maxs": maxs, "s": info["s"], "L": info["L"],
                    "M": M, "direction": "L2C" if d == 0 else "C2L", "form": "plain-fp64",
                    "launch": "hip-graph replay" if args.graph else "direct",
+                   "settle_s": args.settle,
                    "inputs_resident": "HBM", "plan_seconds": round(plan_s, 4)},
         "max_rel_err": {"vs_oracle_fmm": err_fmm, "vs_dd_direct_rows": err_direct,
                         "rows_checked": int(rows.size), "tol": tol},
@@ -805,6 +923,7 @@ def main():
         "lines_per_step": lines,
         "batched": batched,
         "dct2": dct,
+        "field": field,
         "at_published_leaf_size": cmp_line,
     }
     print(json.dumps(line), flush=True)
@@ -818,6 +937,8 @@ def main():
     if lines is not None and not lines["max_rel_err_vs_oracle_first"] <= tol:
         bad = True
     if batched is not None and not batched["max_rel_err_vs_oracle"] <= 1e-14 * math.log2(8192):
+        bad = True
+    if field is not None and not field["ok"]:
         bad = True
     if dct is not None and not (dct["max_rel_err_vs_oracle"] <= dct["tol"]
                                 and dct["batched"]["max_rel_err_vs_oracle_rows"] <= 1e-15 * math.log2(2048)

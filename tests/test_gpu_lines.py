@@ -181,3 +181,51 @@ def test_direct_rows_lines_over_grid_z(amd, orc, monkeypatch, direction):
     op = orc.FMMPlan(N, 1025, 18, orc.BOTH)
     for line in range(P):
         assert relerr(g[line], op.execute(np.ascontiguousarray(x[line]), direction)) <= tol(N), line
+
+
+def test_field_256_cube_with_vectors(amd, orc):
+    """The 256^3 field of the benchmark (c_ijk = U(-1,1)/((i+1)(j+1)(k+1)))
+    through L2C along axes 2, 1, 0 (65,536 strided lines per axis, one call
+    each) and C2L back along 0, 1, 2, while 64 vectors of N = 2^18 run L2C on
+    a batch plan's own streams at the same time: 64 sampled lines of every
+    L2C pass against the oracle's single transform, the round trip, and the
+    first and last vector against the oracle."""
+    n, nv, nvec = 256, 64, 1 << 18
+    announce(f"field {n}^3 round trip with {nv} x 2^18 vectors")
+    i, j, k = np.meshgrid(*[np.arange(n, dtype=np.float64)] * 3, indexing="ij")
+    x = (amd.splitmix_uniform(n ** 3, 3).reshape(n, n, n) / ((i + 1) * (j + 1) * (k + 1)))
+    del i, j, k
+    fp = amd.FMMPlan(n, 64, 18, amd.BOTH)
+    op = orc.FMMPlan(n, 64, 18, orc.BOTH)
+    a, b = amd.DeviceBuffer(x.size), amd.DeviceBuffer(x.size)
+    a.upload(x.ravel())
+    bp = amd.BatchPlan([nvec] * nv, 18, amd.L2C, tune=False)
+    uv = amd.packed_coefficients([nvec] * nv, seed=4)
+    vin, vout = amd.DeviceBuffer(nv * nvec), amd.DeviceBuffer(nv * nvec)
+    vin.upload(uv)
+    maps = {2: (n * n, n, 0, 1), 1: (n, 1, n * n, n), 0: (n * n, 1, 0, n * n)}
+    bp.execute_device(vin.ptr, vout.ptr, 0)  # the batch streams run beside the field passes
+    rng = np.random.default_rng(12)
+    src, dst, cur = a, b, x
+    for ax in (2, 1, 0):
+        fp.execute_lines_device(src.ptr, dst.ptr, 0, n * n, *maps[ax])
+        fp.synchronize()
+        g = dst.download().reshape(n, n, n)
+        gm, cm = np.moveaxis(g, ax, -1), np.moveaxis(cur, ax, -1)
+        for _ in range(64):
+            idx = tuple(int(v) for v in rng.integers(0, n, 2))
+            ref = op.execute(np.ascontiguousarray(cm[idx]), 0)
+            assert relerr(gm[idx], ref) <= tol(n), (ax, idx)
+        src, dst, cur = dst, src, g
+    for ax in (0, 1, 2):
+        fp.execute_lines_device(src.ptr, dst.ptr, 1, n * n, *maps[ax])
+        src, dst = dst, src
+    fp.synchronize()
+    bp.synchronize()
+    back = src.download().reshape(n, n, n)
+    assert float(np.max(np.abs(back - x))) < 1e-10
+    gv = vout.download()
+    ov = orc.FMMPlan(nvec, 64, 18, 0)
+    for t in (0, nv - 1):
+        ref = ov.execute(uv[t * nvec:(t + 1) * nvec], 0)
+        assert relerr(gv[t * nvec:(t + 1) * nvec], ref) <= tol(nvec), t
