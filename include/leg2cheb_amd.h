@@ -251,10 +251,17 @@ typedef struct {
 size_t l2c_batch_num_classes(const l2c_batch_plan *bp);
 int l2c_batch_get_class(const l2c_batch_plan *bp, size_t i, l2c_batch_class *out);
 
-/* Precision form of a plan.  0 = plain FP64 (default).  1 = split form: the
- * M2L matrices are stored per coefficient as FP64, FP32, BF16 or dropped,
+/* Precision form of a plan.  0 = plain FP64 (default).  1 = split storage:
+ * the M2L matrices are stored per coefficient as FP64, FP32, BF16 or dropped,
  * chosen against 2^-29, 2^-44 and 2^-53 of each block's largest coefficient;
- * all arithmetic stays FP64.  Any other value returns -1 with an error text. */
+ * all arithmetic stays FP64.  2 = split products (M = 18): the M2L products
+ * of the blocks below the top levels run as BF16 slices on the matrix cores
+ * (each operand scaled by powers of two and cut into 6 slices of 8 bits,
+ * every anti-diagonal of slice pairs accumulated exactly in FP32,
+ * recombined in FP64); the Lambda tables, the near-field band, the leaf
+ * products, the L2L passes and the final accumulation stay FP64.  Form 2
+ * builds its slice tables on first use (about 4.7x the bytes of the FP64
+ * M2L matrices).  Any other value returns -1 with an error text. */
 int l2c_set_precision(fmm_plan *plan, int form);
 
 /* Length-N DCT-II and DCT-III of real FP64 vectors on the GPU: the steps

@@ -526,6 +526,240 @@ __global__ void __launch_bounds__(kM2lNT) k_m2l(Geo g, const double *__restrict_
   }
 }
 
+// ---------------------------------------------------------------------------
+// k_m2l_mma: M2L of the blocks at levels >= lu for many lines of one plan on
+// MFMA, one block per wave, the lines in groups of 8 (16 MFMA columns:
+// column n = line (n >> 1), parity n & 1).  A block's three submatrices are
+// applied as one stacked operator E (36 x 36): rows 0..17 = [A(b,0) A(b,1)]
+// give the M2L vector of chunk 2b, rows 18..35 = [0 A(b,2)] that of chunk
+// 2b+1, both from the stacked multipoles S = [w(2b+2); w(2b+3)]
+// (src/C/leg2cheb.c:1419-1435).
+//   FORM 0: FP64, v_mfma_f64_16x16x4f64 (A lane l: E[l%16][l/16], B lane l:
+//           S[l/16][l%16], D lane l reg v: [l/16 + 4v][l%16]); E's fragments
+//           stay in registers across the wave's line groups.
+//   FORM 2: BF16 slices, v_mfma_f32_16x16x32_bf16 (A lane l elem e:
+//           E[l%16][8(l/16)+e], B lane l elem e: S[8(l/16)+e][l%16], D lane l
+//           reg v: [4(l/16)+v][l%16]).  Rows of E carry a power-of-two scale
+//           from plan time, columns of S one from their largest entry; both
+//           are cut into kSlices slices q_i 2^(-8(i+1)) with integers
+//           |q_i| <= 256, so every slice is exact in BF16, every product of
+//           slices exact in FP32, and the sum over the 36 k of the pairs
+//           (i, j) on one anti-diagonal i + j = d exact in one FP32
+//           accumulator; anti-diagonals d < kSlices are kept and recombined
+//           in FP64.
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+typedef float f4 __attribute__((ext_vector_type(4)));
+
+// Cut x (|x| < 1) into kSlices integers q_i with x ~ sum q_i 2^(-8(i+1)).
+__device__ __forceinline__ void cut_slices(double x, float q[kSlices]) {
+#pragma unroll
+  for (int i = 0; i < kSlices; i++) {
+    const double sc = (double)(1ull << (8 * (i + 1)));
+    const double qi = rint(x * sc);
+    q[i] = (float)qi;
+    x -= qi / sc;  // exact: qi / sc is x rounded to a coarser grid
+  }
+}
+
+// Plan time (form 2): block B = ntop + blockIdx.x, one wave; lane l writes its
+// fragment elements of every (row tile, k-step, slice) and, for l < 48, the
+// scale of row l.
+__global__ void __launch_bounds__(64) k_slice_A(const double *__restrict__ A, int M, int L, int lu,
+                                                unsigned short *__restrict__ q,
+                                                double *__restrict__ sc) {
+  const int B = (int)total_blocks(lu) + blockIdx.x, l = threadIdx.x;
+  int lev = lu;
+  while ((int)total_blocks(lev + 1) <= B) lev++;
+  const int b = B - (int)total_blocks(lev);
+  const size_t MM = (size_t)M * M;
+  const double *Ab = A + (3 * total_blocks(lev) + 3 * (size_t)b) * MM;
+  auto E = [&](int r, int k) -> double {
+    if (r < M) return k < M ? Ab[r * M + k] : (k < 2 * M ? Ab[MM + r * M + (k - M)] : 0.0);
+    if (r < 2 * M && k >= M && k < 2 * M) return Ab[2 * MM + (r - M) * M + (k - M)];
+    return 0.0;
+  };
+  auto row_scale = [&](int r) -> double {
+    double m = 0.0;
+    for (int k = 0; k < 2 * M; k++) m = fmax(m, fabs(E(r, k)));
+    if (m == 0.0) return 0.0;
+    int e;
+    (void)frexp(m, &e);
+    return ldexp(1.0, e);  // m < 2^e
+  };
+  const size_t blk = (size_t)blockIdx.x;
+  if (l < 48) sc[blk * 48 + l] = row_scale(l);
+  for (int t = 0; t < 3; t++) {
+    const int r = 16 * t + (l & 15);
+    const double s_r = row_scale(r);
+    for (int kk = 0; kk < 2; kk++) {
+      float qs[8][kSlices];
+#pragma unroll
+      for (int e = 0; e < 8; e++) {
+        const int k = 32 * kk + 8 * (l >> 4) + e;
+        cut_slices(s_r > 0 ? E(r, k) / s_r : 0.0, qs[e]);
+      }
+      for (int i = 0; i < kSlices; i++) {
+        unsigned short *dst = q + ((((blk * 3 + t) * 2 + kk) * kSlices + i) * 64 + l) * 8;
+#pragma unroll
+        for (int e = 0; e < 8; e++) dst[e] = (unsigned short)(__float_as_uint(qs[e][i]) >> 16);
+      }
+    }
+  }
+}
+
+template <int FORM>
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) k_m2l_mma(Geo g, const double *__restrict__ A,
+                                                 const bf16x8 *__restrict__ Aq,
+                                                 const double *__restrict__ Asc,
+                                                 double *__restrict__ wsall, size_t wsstride,
+                                                 size_t nlines, int groups_per_wg) {
+  extern __shared__ double sm[];
+  constexpr int M = 18, M2 = 36, MM = 324, Mp = 19;
+  const int tid = threadIdx.x, lane = tid & 63;
+  const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
+  const int ntop = (int)total_blocks(g.lu), ntot = (int)total_blocks(g.L);
+  const int B = ntop + (int)blockIdx.x * 4 + wave;
+  if (B >= ntot) return;  // no work-group barrier in this kernel
+  int lev = g.lu;
+  while ((int)total_blocks(lev + 1) <= B) lev++;
+  const int b = B - (int)total_blocks(lev);
+  const WsLayout lay = ws_layout(g);
+  const size_t sofs = lay.w + (slot_base(lev) + 2 * (size_t)b + 2) * M2;  // S: w(2b+2), w(2b+3)
+  const size_t oofs = lay.m2l + (slot_base(lev) + 2 * (size_t)b) * M2;    // out: chunks 2b, 2b+1
+  const size_t ngroups = (nlines + 7) / 8;
+  const size_t gr0 = (size_t)blockIdx.y * groups_per_wg;
+  const size_t gr1 = std::min(ngroups, gr0 + groups_per_wg);
+  const int n = lane & 15;
+  if (FORM == 0) {
+    // the block into this wave's LDS rows, then E's 23 fragments into registers
+    double *Al = sm + wave * 3 * M * Mp;
+    const double *Ab = A + (size_t)(3 * total_blocks(lev) + 3 * b) * MM;
+    constexpr int QA = (3 * MM + 63) / 64;
+    double va[QA];
+#pragma unroll
+    for (int q = 0; q < QA; q++) {
+      const int e = q * 64 + lane;
+      va[q] = e < 3 * MM ? Ab[e] : 0.0;
+    }
+#pragma unroll
+    for (int q = 0; q < QA; q++) {
+      const int e = q * 64 + lane;
+      if (e < 3 * MM) Al[(e / M) * Mp + e % M] = va[q];
+    }
+    wave_sync();
+    auto E = [&](int r, int k) -> double {
+      if (r < M) return k < M ? Al[r * Mp + k] : Al[(M + r) * Mp + (k - M)];
+      if (r < 2 * M && k >= M) return Al[(M + r) * Mp + (k - M)];  // A(b,2) rows
+      return 0.0;
+    };
+    double ea[23];
+#pragma unroll
+    for (int j = 0; j < 9; j++) {
+      const int k = 4 * j + (lane >> 4);
+      ea[j] = E(n, k);
+      ea[9 + j] = E(16 + n, k);
+      if (j >= 4) ea[18 + j - 4] = E(32 + n, k);
+    }
+    for (size_t gr = gr0; gr < gr1; gr++) {
+      const size_t line = gr * 8 + (n >> 1);
+      const bool live = line < nlines;
+      const double *S = wsall + (live ? line : 0) * wsstride + sofs + (n & 1) * M;
+      double bv[9];
+#pragma unroll
+      for (int j = 0; j < 9; j++) {
+        const int k = 4 * j + (lane >> 4);
+        bv[j] = live ? S[(k >= M ? M2 : 0) + (k >= M ? k - M : k)] : 0.0;
+      }
+      d4 D0 = {0, 0, 0, 0}, D1 = D0, D2 = D0;
+#pragma unroll
+      for (int j = 0; j < 9; j++) {
+        D0 = __builtin_amdgcn_mfma_f64_16x16x4f64(ea[j], bv[j], D0, 0, 0, 0);
+        D1 = __builtin_amdgcn_mfma_f64_16x16x4f64(ea[9 + j], bv[j], D1, 0, 0, 0);
+        if (j >= 4) D2 = __builtin_amdgcn_mfma_f64_16x16x4f64(ea[18 + j - 4], bv[j], D2, 0, 0, 0);
+      }
+      if (live) {
+        double *o = wsall + line * wsstride + oofs + (n & 1) * M;
+#pragma unroll
+        for (int v = 0; v < 4; v++) {
+          const int r0 = (lane >> 4) + 4 * v;  // rows of tiles 0, 1, 2: r0, 16 + r0, 32 + r0
+          o[r0] = D0[v];                                          // r0 < 16: chunk 2b
+          o[16 + r0 < M ? 16 + r0 : M2 + 16 + r0 - M] = D1[v];     // 16..17: 2b; 18..31: 2b+1
+          if (32 + r0 < 2 * M) o[M2 + 32 + r0 - M] = D2[v];         // 32..35: 2b+1
+        }
+      }
+    }
+  } else {
+    const size_t blk = (size_t)(B - ntop);
+    const bf16x8 *Ablk = Aq + blk * 3 * 2 * kSlices * 64;
+    for (size_t gr = gr0; gr < gr1; gr++) {
+      const size_t line = gr * 8 + (n >> 1);
+      const bool live = line < nlines;
+      const double *S = wsall + (live ? line : 0) * wsstride + sofs + (n & 1) * M;
+      // this lane's entries of S: k = 8 (l/16) + e (k-step 0), 32 + 8 (l/16) + e (k-step 1)
+      double sv[2][8];
+      double mx = 0.0;
+#pragma unroll
+      for (int kk = 0; kk < 2; kk++)
+#pragma unroll
+        for (int e = 0; e < 8; e++) {
+          const int k = 32 * kk + 8 * (lane >> 4) + e;
+          sv[kk][e] = (live && k < M2) ? S[(k >= M ? M2 : 0) + (k >= M ? k - M : k)] : 0.0;
+          mx = fmax(mx, fabs(sv[kk][e]));
+        }
+      mx = fmax(mx, __shfl_xor(mx, 16));
+      mx = fmax(mx, __shfl_xor(mx, 32));
+      double s_c = 0.0;
+      if (mx > 0) {
+        int ex;
+        (void)frexp(mx, &ex);
+        s_c = ldexp(1.0, ex);
+      }
+      // the column's slices as B fragments
+      bf16x8 bq[2][kSlices];
+#pragma unroll
+      for (int kk = 0; kk < 2; kk++)
+#pragma unroll
+        for (int e = 0; e < 8; e++) {
+          float qs[kSlices];
+          cut_slices(s_c > 0 ? sv[kk][e] / s_c : 0.0, qs);
+#pragma unroll
+          for (int i = 0; i < kSlices; i++) bq[kk][i][e] = (__bf16)qs[i];
+        }
+      // one row tile at a time: anti-diagonal accumulators, A's slices
+      // fetched per slice (L2-resident), the column's slices kept
+      double *o = wsall + line * wsstride + oofs + (n & 1) * M;
+      const double *scb = Asc + blk * 48;
+#pragma unroll 1
+      for (int t = 0; t < 3; t++) {
+        f4 acc[kSlices];
+#pragma unroll
+        for (int d = 0; d < kSlices; d++) acc[d] = (f4){0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+        for (int i = 0; i < kSlices; i++) {
+          const bf16x8 a0 = Ablk[((t * 2 + 0) * kSlices + i) * 64 + lane];
+          const bf16x8 a1 = Ablk[((t * 2 + 1) * kSlices + i) * 64 + lane];
+#pragma unroll
+          for (int j = 0; j + i < kSlices; j++) {
+            acc[i + j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a0, bq[0][j], acc[i + j], 0, 0, 0);
+            acc[i + j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a1, bq[1][j], acc[i + j], 0, 0, 0);
+          }
+        }
+        if (live) {
+#pragma unroll
+          for (int v = 0; v < 4; v++) {
+            const int r = 16 * t + 4 * (lane >> 4) + v;
+            if (r >= M2) continue;
+            double val = 0.0;
+#pragma unroll
+            for (int d = kSlices - 1; d >= 0; d--) val += ldexp((double)acc[d][v], -8 * (d + 2));
+            o[r < M ? r : M2 + r - M] = val * (scb[r] * s_c);
+          }
+        }
+      }
+    }
+  }
+}
+
 // Near-field band for four consecutive half-rows r0..r0+3 of one parity:
 // columns c = r0 + t, t < len (len equal for the four rows), term
 //   L2C: (a[t-rho] a[r+c+par]) x[c]            (src/C/leg2cheb.c:259,263)
@@ -2302,8 +2536,33 @@ static int launch_fmm_m(fmm_plan_s *p, const double *d_in, double *d_out, size_t
     const DevTables::Split &sp = p->t.sp[dir];
     const SplitView sv{p->precision == 1 ? sp.rec : nullptr, sp.pos, sp.n64, sp.n32, sp.n16,
                        sp.rec_bytes};
-    k_m2l<MT><<<gm, kM2lNT, m2l_shared_bytes(g), st>>>(g, p->t.A[dir], ws, wsstride, bpw,
-                                                      nlines, lpb, sv);
+    const bool mma = MT == 18 && (p->precision == 2 ||
+                                  (p->m2l_mma_lines > 0 && nlines >= (size_t)p->m2l_mma_lines));
+    if (!mma) {
+      k_m2l<MT><<<gm, kM2lNT, m2l_shared_bytes(g), st>>>(g, p->t.A[dir], ws, wsstride, bpw,
+                                                        nlines, lpb, sv);
+    } else {
+      // the blocks above lu (multipoles summed from the k_up contributions)
+      // as before; the rest on MFMA
+      if (ntop > 0)
+        k_m2l<MT><<<dim3((unsigned)ntop, gm.y), kM2lNT, m2l_shared_bytes(g), st>>>(
+            g, p->t.A[dir], ws, wsstride, bpw, nlines, lpb, sv);
+      const int nblk = ntot - ntop;
+      const unsigned mx = (unsigned)((nblk + 3) / 4);
+      const size_t ngroups = (nlines + 7) / 8;
+      // line groups per work-group: one while the grid stays small, more so
+      // that about 8192 work-groups share the lines
+      const size_t gpw = std::max<size_t>(1, (mx * ngroups + 8191) / 8192);
+      const dim3 gmm(mx, (unsigned)((ngroups + gpw - 1) / gpw));
+      if (p->precision == 2) {
+        const DevTables::Slices &sl = p->t.sl[dir];
+        k_m2l_mma<2><<<gmm, 256, 0, st>>>(g, p->t.A[dir], (const bf16x8 *)sl.q, sl.sc, ws,
+                                          wsstride, nlines, (int)gpw);
+      } else {
+        k_m2l_mma<0><<<gmm, 256, 4 * 3 * 18 * 19 * sizeof(double), st>>>(
+            g, p->t.A[dir], nullptr, nullptr, ws, wsstride, nlines, (int)gpw);
+      }
+    }
     prof_end(p->prof, st);
   }
   if (split) {
@@ -2383,6 +2642,24 @@ int run_packed(fmm_plan_s *p, const direct_plan_s *dp, const double *d_in, doubl
       if (launch_fmm(p, d_in, d_out, dir, lm, nl, st, 0)) return -1;
     }
   }
+  return 0;
+}
+
+// Precision form 2 tables of direction dir: the BF16 slices and row scales of
+// every block at levels >= lu (k_slice_A).
+int build_slices(fmm_plan_s *p, int dir) {
+  const Geo &g = p->geo;
+  DevTables::Slices &sl = p->t.sl[dir];
+  if (sl.q) return 0;
+  const size_t nblk = total_blocks(g.L) - total_blocks(g.lu);
+  if (nblk == 0) return 0;
+  const size_t nq = nblk * 3 * 2 * kSlices * 64 * 8;
+  L2C_HIP(hipMalloc(&sl.q, nq * sizeof(unsigned short)), -1);
+  L2C_HIP(hipMalloc(&sl.sc, nblk * 48 * sizeof(double)), -1);
+  p->hbm_bytes += nq * sizeof(unsigned short) + nblk * 48 * sizeof(double);
+  k_slice_A<<<(unsigned)nblk, 64, 0, p->stream>>>(p->t.A[dir], g.M, g.L, g.lu, sl.q, sl.sc);
+  L2C_HIP(hipGetLastError(), -1);
+  L2C_HIP(hipStreamSynchronize(p->stream), -1);
   return 0;
 }
 

@@ -147,7 +147,18 @@ struct DevTables {
     unsigned short *pos = nullptr;
     int n64 = 0, n32 = 0, n16 = 0, rec_bytes = 0;
   } sp[2];
+  // Sliced form of the M2L matrices (precision form 2), per direction, for
+  // the blocks at levels >= lu: each block's stacked operator E (36 x 36:
+  // rows 0..17 [A0 A1], rows 18..35 [0 A2]) with rows scaled by powers of
+  // two sc (48 per block, rows past 35 zero) and cut into kSlices BF16
+  // slices of 8 bits each, stored as MFMA A-operand fragments
+  // [row tile 3][k-step 2][slice][lane 64][8 bf16].
+  struct Slices {
+    unsigned short *q = nullptr;
+    double *sc = nullptr;
+  } sl[2];
 };
+constexpr int kSlices = 6;
 
 // Per-launch HIP-event timing of the execute kernels (bench support).
 enum KernelId {
@@ -193,7 +204,13 @@ struct fmm_plan_s {
   size_t N = 0, maxs = 0, M = 0, direction = 0, lagrange = 0, precompute = 0;
   size_t Nn = 0, L = 0, s = 0, Nc = 0, Nb = 0;
   int device = 0;
-  int precision = 0;  // 0 plain FP64, 1 split form of the M2L matrices
+  // 0 plain FP64; 1 split storage of the M2L matrices; 2 the M2L products of
+  // the blocks at levels >= lu as BF16 slices on MFMA (FP32 accumulation,
+  // FP64 recombination)
+  int precision = 0;
+  // M2L of levels >= lu on FP64 MFMA for calls of at least this many lines
+  // (0: never); L2C_M2L_MMA sets it at plan creation
+  int m2l_mma_lines = 0;
   // downward pass of s = 16/32/64 plans: 0 the fused k_down8, 1 k_band +
   // k_far, 2 k_band + k_far for calls of more than one line
   int down_split = 0;
@@ -254,6 +271,7 @@ int run_lines(fmm_plan_s *p, const double *d_in, double *d_out, size_t dir, size
 int run_packed(fmm_plan_s *p, const direct_plan_s *dp, const double *d_in, double *d_out,
                size_t dir, size_t nlines, const size_t *d_off, const int *d_n, hipStream_t st);
 int reserve_lines(fmm_plan_s *p, size_t nlines);
+int build_slices(fmm_plan_s *p, int dir);  // precision form 2 tables of one direction
 void drop_graphs(fmm_plan_s *p);  // synchronises the plan's stream first
 size_t flops_fmm(const fmm_plan_s *p, size_t dir);
 size_t down_shared_bytes(const Geo &g);  // of the downward kernel g selects

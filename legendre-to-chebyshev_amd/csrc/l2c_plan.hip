@@ -510,6 +510,8 @@ static void destroy_fmm(fmm_plan_s *p) {
   for (int d = 0; d < 2; d++) {
     dfree(p->t.sp[d].rec);
     dfree(p->t.sp[d].pos);
+    dfree(p->t.sl[d].q);
+    dfree(p->t.sl[d].sc);
   }
   dfree(p->t.T);
   dfree(p->t.U);
@@ -585,6 +587,7 @@ fmm_plan_s *create_fmm_impl(size_t N, size_t maxs, size_t M, size_t direction, s
   p->lagrange = lagrange;
   p->precompute = precompute;
   if (const char *e = getenv("L2C_DOWN_SPLIT")) p->down_split = std::max(0, std::min(2, atoi(e)));
+  if (const char *e = getenv("L2C_M2L_MMA")) p->m2l_mma_lines = std::max(0, atoi(e));
   (void)hipGetDevice(&p->device);
   if (hipStreamCreateWithFlags(&p->stream, hipStreamNonBlocking) != hipSuccess) {
     set_error("hipStreamCreate failed");
@@ -1060,8 +1063,13 @@ int l2c_synchronize(fmm_plan *p) {
 }
 int l2c_set_precision(fmm_plan *p, int form) {
   clear_error();
-  if (!p || (form != 0 && form != 1)) {
-    set_error("l2c_set_precision: form must be 0 (plain FP64) or 1 (split M2L matrices)");
+  if (!p || form < 0 || form > 2) {
+    set_error("l2c_set_precision: form must be 0 (plain FP64), 1 (split M2L matrices) or 2 "
+              "(M2L as BF16 slices on MFMA)");
+    return -1;
+  }
+  if (form == 2 && p->L >= 1 && p->M != 18) {
+    set_error("l2c_set_precision: form 2 needs M = 18");
     return -1;
   }
   drop_graphs(p);  // their launches carry the previous form's arguments
@@ -1069,6 +1077,11 @@ int l2c_set_precision(fmm_plan *p, int form) {
     L2C_HIP(hipStreamSynchronize(p->stream), -1);
     for (int d = 0; d < 2; d++)
       if (p->t.A[d] && !p->t.sp[d].rec && build_split(p, d)) return -1;
+  }
+  if (form == 2 && p->L >= 1) {
+    L2C_HIP(hipStreamSynchronize(p->stream), -1);
+    for (int d = 0; d < 2; d++)
+      if (p->t.A[d] && build_slices(p, d)) return -1;
   }
   p->precision = form;
   return 0;
