@@ -799,6 +799,59 @@ def main():
             log("[bench] split form error above tolerance")
             split["failed"] = True
 
+    # precision form 2 (the M2L products below the top levels as BF16 slices
+    # on MFMA), single transforms and 8 lines per step, beside form 0
+    form2 = None
+    if args.split_steps > 0:
+        plan.set_precision(2)
+        for _ in range(args.warmup):
+            plan.execute_device(din.ptr, dout.ptr, d)
+        plan.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(args.split_steps):
+            plan.execute_device(din.ptr, dout.ptr, d)
+        plan.synchronize()
+        el2 = time.perf_counter() - t0
+        plan.profile(1)
+        for _ in range(20):
+            plan.execute_device(din.ptr, dout.ptr, d)
+        plan.synchronize()
+        prof2 = plan.profile_read()
+        plan.profile(False)
+        g2 = dout.download()
+        e2o = float(np.max(np.abs(g2 - o)) / np.max(np.abs(o)))
+        e2d = float(np.max(np.abs(g2[rows] - ref)) / scale)
+        form2 = {"form": "M2L below the top levels as 6 x 6 BF16 slices on MFMA, exact FP32 "
+                         "anti-diagonal sums, FP64 recombination",
+                 "value": round(args.split_steps / el2, 2), "unit": "transforms/s",
+                 "ms_per_step": round(1e3 * el2 / args.split_steps, 5),
+                 "vs_form0": round((args.split_steps / el2) / value, 3),
+                 "kernels_us": {k: round(1e3 * ms / max(c, 1), 3) for k, (ms, c) in prof2.items()},
+                 "max_rel_err": {"vs_oracle_fmm": e2o, "vs_dd_direct_rows": e2d, "tol": 1e-13}}
+        if args.lines > 1:
+            B = args.lines
+            ub = np.concatenate([amd.coefficients(N, seed=1 + b, decay=args.decay) for b in range(B)])
+            bi, bo = amd.DeviceBuffer(B * N), amd.DeviceBuffer(B * N)
+            bi.upload(ub)
+            for _ in range(3):
+                plan.execute_lines_device(bi.ptr, bo.ptr, d, B, B, N, 0, 1)
+            plan.synchronize()
+            ls = max(5, args.steps // 8)
+            t0 = time.perf_counter()
+            for _ in range(ls):
+                plan.execute_lines_device(bi.ptr, bo.ptr, d, B, B, N, 0, 1)
+            plan.synchronize()
+            el_l2 = time.perf_counter() - t0
+            gb = bo.download()
+            form2["lines"] = {"vectors_per_step": B, "value": round(B * ls / el_l2, 2),
+                              "ms_per_step": round(1e3 * el_l2 / ls, 4),
+                              "max_rel_err_vs_oracle_first": float(np.max(np.abs(gb[:N] - o)) / np.max(np.abs(o)))}
+            del bi, bo
+        plan.set_precision(0)
+        log(f"[bench] form 2: {form2}")
+        if not (e2o <= 1e-13 and e2d <= 1e-13):
+            form2["failed"] = True
+
     # the drop-in execute() on host arrays: staged through pinned buffers,
     # PCIe included (reported beside the value, never as the value)
     pcie = None
@@ -920,6 +973,7 @@ def main():
         "cpu_baseline": cpu,
         "host_pointer_transforms_s": pcie,
         "split_form": split,
+        "form2_split_products": form2,
         "lines_per_step": lines,
         "batched": batched,
         "dct2": dct,
@@ -933,6 +987,8 @@ def main():
     if cmp_line is not None and not cmp_line["max_rel_err_vs_oracle"] <= tol:
         bad = True
     if split is not None and split.get("failed"):
+        bad = True
+    if form2 is not None and form2.get("failed"):
         bad = True
     if lines is not None and not lines["max_rel_err_vs_oracle_first"] <= tol:
         bad = True
