@@ -241,13 +241,25 @@ int l2c_batch_synchronize(l2c_batch_plan *bp);
 
 /* One size class of a batch plan: its padded length, the number of transforms
  * in it, the method chosen (0 direct, 1 FMM), the FMM leaf size s and depth
- * L, and the tuner's time for the class in milliseconds (-1 if untuned). */
+ * L, the tuner's time for the class in milliseconds (-1 if untuned), the
+ * kernel variant (FMM: 0 fused downward pass, 1 split into band and far
+ * field; direct: 0 dense kernel by line count, 1 one tile of all rows, 2 row
+ * tiles) and whether the choice came from the tuning cache. */
 typedef struct {
   size_t n_max, count;
   int method;
   size_t s, L;
   double ms;
+  int variant;
+  int cached;
 } l2c_batch_class;
+/* Tuning cache for l2c_create_batch(..., tune = 1): a text file holding the
+ * tuner's choice per (device, size class, population, direction, M); classes
+ * found there skip the timing, new choices are appended.  path = NULL turns
+ * it off.  The environment variable L2C_TUNE_CACHE gives the default path.
+ * No counterpart in the reference (its CLI keeps FFTW wisdom, src/bin/l2c.c:
+ * 417-454).  Returns 0. */
+int l2c_tune_cache(const char *path);
 size_t l2c_batch_num_classes(const l2c_batch_plan *bp);
 int l2c_batch_get_class(const l2c_batch_plan *bp, size_t i, l2c_batch_class *out);
 

@@ -40,7 +40,7 @@ class PlanInfo(C.Structure):
 
 class BatchClass(C.Structure):
     _fields_ = [("n_max", _sz), ("count", _sz), ("method", C.c_int), ("s", _sz), ("L", _sz),
-                ("ms", C.c_double)]
+                ("ms", C.c_double), ("variant", C.c_int), ("cached", C.c_int)]
 
 
 _lib = None
@@ -125,6 +125,8 @@ def lib():
         L.l2c_execute_batch_device.argtypes = [vp, vp, vp, _sz, vp]
         L.l2c_execute_batch.restype = C.c_int
         L.l2c_execute_batch.argtypes = [vp, vp, vp, _sz]
+        L.l2c_tune_cache.restype = C.c_int
+        L.l2c_tune_cache.argtypes = [C.c_char_p]
         L.l2c_batch_graph.restype = C.c_int
         L.l2c_batch_graph.argtypes = [vp, C.c_int]
         L.l2c_batch_synchronize.restype = C.c_int
@@ -427,7 +429,7 @@ class BatchPlan:
                 raise L2CError(last_error())
             out.append({"n_max": c.n_max, "count": c.count,
                         "method": "direct" if c.method == 0 else "fmm", "s": c.s, "L": c.L,
-                        "ms": c.ms})
+                        "ms": c.ms, "variant": c.variant, "cached": bool(c.cached)})
         return out
 
     def split(self, packed):
@@ -532,6 +534,11 @@ class DeviceBuffer:
 
     def __del__(self):
         self.free()
+
+
+def tune_cache(path):
+    """Set (or with None clear) the batch tuner's cache file (l2c_tune_cache)."""
+    lib().l2c_tune_cache(None if path is None else str(path).encode())
 
 
 def splitmix_uniform(n, seed, lo=-1.0, hi=1.0):

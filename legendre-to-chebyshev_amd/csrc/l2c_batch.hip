@@ -21,7 +21,12 @@
 
 #include <algorithm>
 #include <cmath>
+#include <cstdlib>
 #include <cstring>
+#include <fstream>
+#include <map>
+#include <mutex>
+#include <string>
 #include <vector>
 
 #include "l2c_internal.h"
@@ -34,10 +39,15 @@ struct BatchClass {
   std::vector<size_t> idx;  // transforms of the class, in batch order
   int method = 0;        // 0 direct, 1 FMM
   size_t s = 0, L = 0;   // FMM leaf size and depth
+  // FMM: 0 fused downward kernel, 1 k_band + k_far; direct: 0 the default
+  // dense kernel for the line count, 1 the one-tile kernel, 2 the tiled one
+  int variant = 0;
   double ms = -1;        // tuner time of the chosen method
   int alt_method = -1;   // the tuner's runner-up (-1: none) and its time
   size_t alt_s = 0;
+  int alt_variant = 0;
   double alt_ms = -1;
+  bool cached = false;   // the choice came from the tuning cache
   double work = 0;       // estimated cost (orders classes over streams)
   int stream = 0;        // stream index the class runs on
   fmm_plan_s *fp = nullptr;
@@ -111,14 +121,18 @@ static int run_class(l2c_batch_plan_s *bp, const BatchClass &c, const double *d_
                     bp->d_n + c.first, st);
 }
 
-// Make the plan for one candidate method of a class (method 0: direct,
-// 1: FMM with leaf size s).  Returns 0 or -1 (candidate not available).
-static int make_candidate(l2c_batch_plan_s *bp, BatchClass &c, int method, size_t s) {
+// Make the plan for one candidate of a class (method 0: direct with dense
+// kernel variant `variant`, 1: FMM with leaf size s and downward variant
+// `variant`).  Returns 0 or -1 (candidate not available).
+static int make_candidate(l2c_batch_plan_s *bp, BatchClass &c, int method, size_t s,
+                          int variant = 0) {
   c.method = method;
   c.s = s;
+  c.variant = variant;
   c.L = 0;
   if (method == 0) {
     c.dp = make_direct(c.n_max, bp->direction, 0);
+    if (c.dp) c.dp->dense_tile = variant;
     return c.dp ? 0 : -1;
   }
   c.fp = create_fmm_impl(c.n_max, s, bp->M, bp->direction, 0, 0, 0);
@@ -127,13 +141,25 @@ static int make_candidate(l2c_batch_plan_s *bp, BatchClass &c, int method, size_
     destroy_class_plans(c);
     return -1;
   }
+  c.fp->down_split = variant;
   c.L = c.fp->L;
   return 0;
 }
 
-// Time the class's current plan on the staging buffers (min of 3 after one
+// Device buffers of the batch's size for the tuner's timings (zeros in).
+static int ensure_tune_bufs(l2c_batch_plan_s *bp) {
+  if (bp->d_in) return 0;
+  const size_t bytes = std::max<size_t>(bp->total, 1) * sizeof(double);
+  L2C_HIP(hipMalloc(&bp->d_in, bytes), -1);
+  L2C_HIP(hipMalloc(&bp->d_out, bytes), -1);
+  L2C_HIP(hipMemsetAsync(bp->d_in, 0, bytes, bp->st[0]), -1);
+  return 0;
+}
+
+// Time the class's current plan on the tuning buffers (min of 3 after one
 // warm-up), milliseconds; < 0 on failure.
 static double time_class(l2c_batch_plan_s *bp, const BatchClass &c, size_t dir) {
+  if (ensure_tune_bufs(bp)) return -1;
   hipStream_t st = bp->st[0];
   hipEvent_t a, b;
   if (hipEventCreate(&a) != hipSuccess) return -1;
@@ -162,13 +188,30 @@ static double time_class(l2c_batch_plan_s *bp, const BatchClass &c, size_t dir) 
   return best;
 }
 
-// Candidate methods of size class k: the direct sum up to 2^13, the FMM with
-// s = 16, 32, 64 where the depth L = k - log2(s) - 2 is at least 1.
-static std::vector<std::pair<int, size_t>> candidates(int k) {
-  std::vector<std::pair<int, size_t>> v;
-  if (k <= 13) v.push_back({0, 0});
+// Candidates of size class k: the direct sum up to 2^13 (for 32 < 2^k <= 256
+// with each dense kernel: one tile of all rows, or row tiles), the FMM with
+// s = 16, 32, 64 where the depth L = k - log2(s) - 2 is at least 1, each with
+// the fused and the split downward pass.
+struct Cand {
+  int method;
+  size_t s;
+  int variant;
+};
+static std::vector<Cand> candidates(int k) {
+  std::vector<Cand> v;
+  if (k <= 13) {
+    if (k > 5 && k <= 8) {
+      v.push_back({0, 0, 1});
+      v.push_back({0, 0, 2});
+    } else {
+      v.push_back({0, 0, 0});
+    }
+  }
   for (int ls = 4; ls <= 6; ls++)
-    if (k - ls - 2 >= 1) v.push_back({1, (size_t)1 << ls});
+    if (k - ls - 2 >= 1) {
+      v.push_back({1, (size_t)1 << ls, 0});
+      v.push_back({1, (size_t)1 << ls, 1});
+    }
   return v;
 }
 
@@ -178,6 +221,7 @@ static int launch_batch(l2c_batch_plan_s *bp, const double *d_in, double *d_out,
 // The whole batch, eager, on the staging buffers (min of 3 after one warm-up),
 // milliseconds; < 0 on failure.
 static double time_batch(l2c_batch_plan_s *bp, size_t dir) {
+  if (ensure_tune_bufs(bp)) return -1;
   hipStream_t st = bp->st[0];
   hipEvent_t a, b;
   if (hipEventCreate(&a) != hipSuccess) return -1;
@@ -214,12 +258,12 @@ static int refine_close_calls(l2c_batch_plan_s *bp, size_t dir) {
   for (auto &c : bp->cls) {
     if (c.alt_method < 0 || c.alt_ms < 0 || c.alt_ms > 1.25 * c.ms) continue;
     const double t_cur = time_batch(bp, dir);
-    const int m0 = c.method;
+    const int m0 = c.method, v0 = c.variant;
     const size_t s0 = c.s;
     destroy_class_plans(c);
-    if (make_candidate(bp, c, c.alt_method, c.alt_s)) {
+    if (make_candidate(bp, c, c.alt_method, c.alt_s, c.alt_variant)) {
       clear_error();
-      if (make_candidate(bp, c, m0, s0)) return -1;
+      if (make_candidate(bp, c, m0, s0, v0)) return -1;
       continue;
     }
     const double t_alt = time_batch(bp, dir);
@@ -227,12 +271,85 @@ static int refine_close_calls(l2c_batch_plan_s *bp, size_t dir) {
       std::swap(c.ms, c.alt_ms);
       c.alt_method = m0;
       c.alt_s = s0;
+      c.alt_variant = v0;
       continue;
     }
     destroy_class_plans(c);
-    if (make_candidate(bp, c, m0, s0)) return -1;
+    if (make_candidate(bp, c, m0, s0, v0)) return -1;
   }
   return 0;
+}
+
+// ---------------------------------------------------------------------------
+// Tuning cache: the tuner's choice per (device, size class, class population,
+// direction, rank), kept in a text file (l2c_tune_cache or L2C_TUNE_CACHE),
+// one "key method s variant ms" line per choice, so that later batch plans
+// of the same shapes skip the timing (the reference keeps FFTW wisdom for the
+// same purpose, src/bin/l2c.c:417-454).
+static std::mutex g_tc_mu;
+static std::string g_tc_path;
+static bool g_tc_path_set = false, g_tc_loaded = false;
+static std::map<std::string, std::pair<Cand, double>> g_tc;
+
+static std::string tune_cache_path() {
+  if (!g_tc_path_set) {
+    g_tc_path_set = true;
+    if (const char *e = getenv("L2C_TUNE_CACHE")) g_tc_path = e;
+  }
+  return g_tc_path;
+}
+
+static std::string tune_key(const l2c_batch_plan_s *bp, const BatchClass &c, size_t dir) {
+  static std::string dev;
+  if (dev.empty()) {
+    hipDeviceProp_t prop;
+    int d = 0;
+    (void)hipGetDevice(&d);
+    if (hipGetDeviceProperties(&prop, d) == hipSuccess)
+      dev = std::string(prop.gcnArchName) + ":" + std::to_string(prop.multiProcessorCount);
+    else
+      dev = "unknown";
+    for (auto &ch : dev)
+      if (ch == ' ') ch = '_';
+  }
+  // the population enters as its power of two (the choice moves with it)
+  size_t pop = 1;
+  while (pop < c.idx.size()) pop <<= 1;
+  return dev + "/n" + std::to_string(c.n_max) + "/count" + std::to_string(pop) + "/dir" +
+         std::to_string(dir) + "/M" + std::to_string(bp->M);
+}
+
+static void tune_cache_load() {  // under g_tc_mu
+  if (g_tc_loaded) return;
+  g_tc_loaded = true;
+  const std::string path = tune_cache_path();
+  if (path.empty()) return;
+  std::ifstream f(path);
+  std::string key;
+  Cand cd{};
+  double ms;
+  while (f >> key >> cd.method >> cd.s >> cd.variant >> ms) g_tc[key] = {cd, ms};
+}
+
+static bool tune_cache_get(const std::string &key, Cand &cd, double &ms) {
+  std::lock_guard<std::mutex> lock(g_tc_mu);
+  if (tune_cache_path().empty()) return false;
+  tune_cache_load();
+  auto it = g_tc.find(key);
+  if (it == g_tc.end()) return false;
+  cd = it->second.first;
+  ms = it->second.second;
+  return true;
+}
+
+static void tune_cache_put(const std::string &key, const Cand &cd, double ms) {
+  std::lock_guard<std::mutex> lock(g_tc_mu);
+  const std::string path = tune_cache_path();
+  if (path.empty()) return;
+  tune_cache_load();
+  g_tc[key] = {cd, ms};
+  std::ofstream f(path, std::ios::app);
+  if (f) f << key << ' ' << cd.method << ' ' << cd.s << ' ' << cd.variant << ' ' << ms << '\n';
 }
 
 // The untuned rule: direct while 2^k <= 4 maxs (maxs = 64: the reference's
@@ -313,12 +430,11 @@ static l2c_batch_plan_s *create_batch_impl(const size_t *n, size_t count, size_t
     }
   }
   // the tuner times the candidates on device buffers of the batch's size
-  // (zeros in, scratch out), released once the methods are chosen; the host
-  // staging of l2c_execute_batch is allocated by its first call
+  // (zeros in, scratch out), made on its first timing and released once the
+  // methods are chosen; the host staging of l2c_execute_batch is allocated by
+  // its first call
   if (hipMalloc(&bp->d_off, count * sizeof(size_t)) != hipSuccess ||
-      hipMalloc(&bp->d_n, count * sizeof(int)) != hipSuccess ||
-      (tune && hipMalloc(&bp->d_in, tot * sizeof(double)) != hipSuccess) ||
-      (tune && hipMalloc(&bp->d_out, tot * sizeof(double)) != hipSuccess)) {
+      hipMalloc(&bp->d_n, count * sizeof(int)) != hipSuccess) {
     set_error("l2c_create_batch: allocation failed");
     destroy_batch(bp);
     return nullptr;
@@ -328,8 +444,6 @@ static l2c_batch_plan_s *create_batch_impl(const size_t *n, size_t count, size_t
           (destroy_batch(bp), nullptr));
   L2C_HIP(hipMemcpyAsync(bp->d_n, hn.data(), count * sizeof(int), hipMemcpyHostToDevice, s0),
           (destroy_batch(bp), nullptr));
-  if (tune)
-    L2C_HIP(hipMemsetAsync(bp->d_in, 0, tot * sizeof(double), s0), (destroy_batch(bp), nullptr));
   L2C_HIP(hipStreamSynchronize(s0), (destroy_batch(bp), nullptr));
 
   // method per class
@@ -345,10 +459,20 @@ static l2c_batch_plan_s *create_batch_impl(const size_t *n, size_t count, size_t
       }
       continue;
     }
+    Cand hit{};
+    double hit_ms = -1;
+    if (tune_cache_get(tune_key(bp, c, tdir), hit, hit_ms)) {
+      if (!make_candidate(bp, c, hit.method, hit.s, hit.variant)) {
+        c.ms = hit_ms;
+        c.cached = true;
+        continue;
+      }
+      clear_error();
+    }
     double best = -1, second = -1;
-    std::pair<int, size_t> pick{-1, 0}, alt{-1, 0};
+    Cand pick{-1, 0, 0}, alt{-1, 0, 0};
     for (const auto &m : candidates(c.k)) {
-      if (make_candidate(bp, c, m.first, m.second)) {
+      if (make_candidate(bp, c, m.method, m.s, m.variant)) {
         clear_error();
         continue;
       }
@@ -365,11 +489,12 @@ static l2c_batch_plan_s *create_batch_impl(const size_t *n, size_t count, size_t
         alt = m;
       }
     }
-    c.alt_method = alt.first;
-    c.alt_s = alt.second;
+    c.alt_method = alt.method;
+    c.alt_s = alt.s;
+    c.alt_variant = alt.variant;
     c.alt_ms = second;
-    if (pick.first < 0 || make_candidate(bp, c, pick.first, pick.second)) {
-      if (pick.first < 0) set_error("l2c_create_batch: no method ran for a size class");
+    if (pick.method < 0 || make_candidate(bp, c, pick.method, pick.s, pick.variant)) {
+      if (pick.method < 0) set_error("l2c_create_batch: no method ran for a size class");
       destroy_batch(bp);
       return nullptr;
     }
@@ -406,7 +531,10 @@ static l2c_batch_plan_s *create_batch_impl(const size_t *n, size_t count, size_t
     destroy_batch(bp);
     return nullptr;
   }
-  if (tune) {
+  if (tune)
+    for (const auto &c : bp->cls)
+      if (!c.cached) tune_cache_put(tune_key(bp, c, tdir), {c.method, c.s, c.variant}, c.ms);
+  if (bp->d_in) {
     L2C_HIP(hipStreamSynchronize(s0), (destroy_batch(bp), nullptr));
     (void)hipFree(bp->d_in);
     (void)hipFree(bp->d_out);
@@ -528,6 +656,16 @@ int l2c_execute_batch(l2c_batch_plan *bp, const double *in, double *out, size_t 
   return 0;
 }
 
+int l2c_tune_cache(const char *path) {
+  clear_error();
+  std::lock_guard<std::mutex> lock(g_tc_mu);
+  g_tc_path = path ? path : "";
+  g_tc_path_set = true;
+  g_tc_loaded = false;
+  g_tc.clear();
+  return 0;
+}
+
 int l2c_batch_graph(l2c_batch_plan *bp, int enable) {
   clear_error();
   if (!bp) {
@@ -568,6 +706,8 @@ int l2c_batch_get_class(const l2c_batch_plan *bp, size_t i, l2c_batch_class *out
   out->s = c.s;
   out->L = c.L;
   out->ms = c.ms;
+  out->variant = c.variant;
+  out->cached = c.cached ? 1 : 0;
   return 0;
 }
 

@@ -2390,7 +2390,8 @@ static int launch_direct_lines(const direct_plan_s *dp, const double *d_in, doub
   // cost per line, the tiled one far more work-groups for a given call: above
   // N = 32 the tiled kernel takes calls of up to kGemmFewLines lines (N = 256:
   // 600 lines in 7 us against 18 us; 65,536 lines in 119 us against 90 us)
-  const bool tiled = N > kDenseMaxN || (N > 32 && nlines <= kGemmFewLines);
+  const bool tiled = N > kDenseMaxN || (dp->dense_tile == 2 && N > 32) ||
+                     (dp->dense_tile == 0 && N > 32 && nlines <= kGemmFewLines);
   if (dp->d_K[dir] && nlines >= 4 && tiled) {
     const size_t nb = (nlines + kGemmLT - 1) / kGemmLT;
     const dim3 grid((unsigned)(2 * nb), (unsigned)((dp->NPP + kGemmRT - 1) / kGemmRT));

@@ -191,6 +191,9 @@ struct direct_plan_s {
   // 2 x NPP x NPP, NPP = ceil(N/2) rounded up to 16
   double *d_K[2] = {nullptr, nullptr};
   int NPP = 0;
+  // dense kernel for line batches: 0 by the line count, 1 one tile of all
+  // rows (k_lines_dense), 2 row tiles (k_lines_gemm); the batch tuner sets it
+  int dense_tile = 0;
   // lines per grid row of k_direct (grid.y limit 65535; L2C_DIRECT_LINES_Y
   // sets a smaller value, which the tests use to reach grid.z > 1 cheaply)
   unsigned lines_y = 65535;

@@ -87,3 +87,33 @@ def test_batch_log_uniform_4096(amd, orc):
     sample = sorted(set(small) | set(rng.choice(4096, 200, replace=False).tolist()))
     w = check(amd, orc, bp, u, g, 0, which=sample)
     print(f"  {len(sample)} transforms checked, worst error / tol = {w:.3e}")
+
+
+def test_tune_cache_round_trip(amd, orc, tmp_path):
+    """The tuner's choices go to the cache file on the first plan and come
+    back from it on the second (no timing): same method, leaf size and kernel
+    variant per class, and bit-identical results."""
+    import time
+    path = tmp_path / "l2c_tune.txt"
+    amd.tune_cache(path)
+    try:
+        lengths = amd.log_uniform_lengths(600, seed=31)
+        announce(f"tune cache {path}")
+        t0 = time.perf_counter()
+        bp1 = amd.BatchPlan(lengths, 18, amd.L2C, tune=True)
+        t1 = time.perf_counter()
+        bp2 = amd.BatchPlan(lengths, 18, amd.L2C, tune=True)
+        t2 = time.perf_counter()
+        c1, c2 = bp1.classes(), bp2.classes()
+        print(f"  plan time cold {1e3 * (t1 - t0):.1f} ms, warm cache {1e3 * (t2 - t1):.1f} ms")
+        assert not any(c["cached"] for c in c1) and all(c["cached"] for c in c2)
+        key = lambda c: (c["n_max"], c["method"], c["s"], c["variant"])
+        assert sorted(map(key, c1)) == sorted(map(key, c2))
+        assert len(path.read_text().splitlines()) == len(c1)
+        u = amd.packed_coefficients(lengths, seed=8)
+        g1, g2 = bp1.execute(u, 0), bp2.execute(u, 0)
+        assert np.array_equal(g1, g2)
+        o = orc.FMMPlan(int(lengths[0]), 64, 18, 0).execute(u[:lengths[0]], 0)
+        assert relerr(g1[:lengths[0]], o) <= 1e-14 * 13
+    finally:
+        amd.tune_cache(None)
