@@ -73,6 +73,9 @@ struct l2c_batch_plan_s {
   const double *g_in = nullptr;
   double *g_out = nullptr;
   size_t g_dir = 99;
+  // classes on the tiled dense product run as one launch (L2C_BATCH_FUSE=0
+  // launches them one by one)
+  bool fuse_dense = true;
   // device buffers of the tuner (plan time only), then the staging of the
   // host-pointer form (allocated by its first call)
   double *d_in = nullptr, *d_out = nullptr;
@@ -379,6 +382,7 @@ static l2c_batch_plan_s *create_batch_impl(const size_t *n, size_t count, size_t
   bp->count = count;
   bp->M = M;
   bp->direction = direction;
+  if (const char *e = getenv("L2C_BATCH_FUSE")) bp->fuse_dense = atoi(e) != 0;
   (void)hipGetDevice(&bp->device);
   bp->n.assign(n, n + count);
   bp->off.resize(count);
@@ -544,13 +548,26 @@ static l2c_batch_plan_s *create_batch_impl(const size_t *n, size_t count, size_t
 }
 
 // Fork from `origin`, run every class on its stream, join back to `origin`.
+// The classes that take the tiled dense product run as one launch (on the
+// stream of the first of them) when fuse_dense is set.
 static int launch_batch(l2c_batch_plan_s *bp, const double *d_in, double *d_out, size_t dir,
                         hipStream_t origin) {
   L2C_HIP(hipEventRecord(bp->fork, origin), -1);
   for (int i = 0; i < kBatchStreams; i++)
     if (bp->st[i] != origin) L2C_HIP(hipStreamWaitEvent(bp->st[i], bp->fork, 0), -1);
-  for (const auto &c : bp->cls)
+  std::vector<DenseItem> fused;
+  int fused_stream = -1;
+  for (const auto &c : bp->cls) {
+    if (bp->fuse_dense && c.dp && (int)fused.size() < 8 && dense_tiled(c.dp, dir, c.idx.size())) {
+      fused.push_back({c.dp, bp->d_off + c.first, bp->d_n + c.first, c.idx.size()});
+      if (fused_stream < 0) fused_stream = c.stream;
+      continue;
+    }
     if (run_class(bp, c, d_in, d_out, dir, bp->st[c.stream])) return -1;
+  }
+  if (!fused.empty() &&
+      run_dense_classes(fused.data(), (int)fused.size(), d_in, d_out, dir, bp->st[fused_stream]))
+    return -1;
   for (int i = 0; i < kBatchStreams; i++) {
     if (bp->st[i] == origin) continue;
     L2C_HIP(hipEventRecord(bp->join[i], bp->st[i]), -1);

@@ -2261,16 +2261,20 @@ static size_t dense_shared_bytes(int NPP) {
 // MFMA operand read stays contiguous
 constexpr int kGemmRT = 64, kGemmLT = 32, kGemmKC = 64, kGemmNT = 256, kGemmLTP = 33;
 constexpr size_t kGemmFewLines = 16384;
+// The work of tile (bx, by) of the tiled dense product (grid (2 x line tiles,
+// row tiles)); k_lines_gemm runs one grid of them, k_lines_gemm_multi the
+// grids of several size classes in one launch.
 template <bool CONTIG>
-__global__ void __launch_bounds__(kGemmNT) __attribute__((amdgpu_waves_per_eu(3)))
-    k_lines_gemm(const double *__restrict__ in, double *__restrict__ out, LineMap lm,
-                 size_t nlines, int N, int NPP, const double *__restrict__ K) {
-  extern __shared__ double sm[];
+__device__ __forceinline__ void lines_gemm_tile(const double *__restrict__ in,
+                                                double *__restrict__ out, const LineMap &lm,
+                                                size_t nlines, int N, int NPP,
+                                                const double *__restrict__ K, unsigned bx,
+                                                unsigned by, double *sm) {
   constexpr int LTP = kGemmLTP, KC = kGemmKC, LT = kGemmLT;
   const int tid = threadIdx.x, lane = tid & 63;
   const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
-  const int rt = blockIdx.y, p = blockIdx.x & 1;
-  const size_t l0 = (size_t)(blockIdx.x >> 1) * LT;
+  const int rt = (int)by, p = bx & 1;
+  const size_t l0 = (size_t)(bx >> 1) * LT;
   size_t *toff = (size_t *)(sm + KC * LTP);
   int *tlen = (int *)(toff + LT);
   if (tid < LT) {
@@ -2367,6 +2371,43 @@ __global__ void __launch_bounds__(kGemmNT) __attribute__((amdgpu_waves_per_eu(3)
     if (r < NPP && 2 * r + p < len[q]) dst[q][(size_t)(2 * r) * es] = sm[lane * LTP + 4 * q + wave];
 }
 
+template <bool CONTIG>
+__global__ void __launch_bounds__(kGemmNT) __attribute__((amdgpu_waves_per_eu(3)))
+    k_lines_gemm(const double *__restrict__ in, double *__restrict__ out, LineMap lm,
+                 size_t nlines, int N, int NPP, const double *__restrict__ K) {
+  extern __shared__ double sm[];
+  lines_gemm_tile<CONTIG>(in, out, lm, nlines, N, NPP, K, blockIdx.x, blockIdx.y, sm);
+}
+
+// Several size classes of an uneven batch (packed, unit-stride lines) on the
+// tiled dense product in one launch: a flat grid over every class's tiles,
+// classes in the order given, each class's tiles row tile by row tile (the
+// long row tiles first, as in k_lines_gemm's grid).
+struct DenseClass {
+  const double *K;
+  const size_t *off;
+  const int *n;
+  size_t nlines;
+  int N, NPP;
+  unsigned bxs, tiles;  // 2 x line tiles; bxs x row tiles
+};
+constexpr int kMaxDenseClasses = 8;
+struct DenseClasses {
+  DenseClass c[kMaxDenseClasses];
+  int count;
+};
+__global__ void __launch_bounds__(kGemmNT) __attribute__((amdgpu_waves_per_eu(3)))
+    k_lines_gemm_multi(const double *__restrict__ in, double *__restrict__ out, DenseClasses dc) {
+  extern __shared__ double sm[];
+  unsigned t = blockIdx.x;
+  int c = 0;
+  while (c + 1 < dc.count && t >= dc.c[c].tiles) t -= dc.c[c++].tiles;
+  const DenseClass &d = dc.c[c];
+  if (t >= d.tiles) return;
+  LineMap lm{1, 0, 0, 1, d.off, d.n};
+  lines_gemm_tile<true>(in, out, lm, d.nlines, d.N, d.NPP, d.K, t % d.bxs, t / d.bxs, sm);
+}
+
 size_t gemm_shared_bytes() {
   return 8 * (size_t)kGemmKC * kGemmLTP + kGemmLT * (sizeof(size_t) + sizeof(int));
 }
@@ -2381,18 +2422,54 @@ static void launch_direct(const direct_plan_s *dp, const double *d_in, double *d
                                                           dp->d_an, dp->d_dn, nlines);
 }
 
+// Whether a call of nlines lines through direct plan dp takes the tiled dense
+// product (k_lines_gemm): the one-tile kernel has the lower cost per line,
+// the tiled one far more work-groups for a given call, so above N = 32 the
+// tiled kernel takes calls of up to kGemmFewLines lines (N = 256: 600 lines
+// in 7 us against 18 us; 65,536 lines in 119 us against 90 us) unless the
+// plan's dense_tile says otherwise.
+bool dense_tiled(const direct_plan_s *dp, size_t dir, size_t nlines) {
+  const int N = (int)dp->N;
+  return dp->d_K[dir] && nlines >= 4 &&
+         (N > kDenseMaxN || (dp->dense_tile == 2 && N > 32) ||
+          (dp->dense_tile == 0 && N > 32 && nlines <= kGemmFewLines));
+}
+
+// Packed unit-stride lines of several direct plans in one tiled dense launch.
+int run_dense_classes(const DenseItem *items, int count, const double *d_in, double *d_out,
+                      size_t dir, hipStream_t st) {
+  DenseClasses dc{};
+  size_t total = 0;
+  for (int i = 0; i < count; i++) {
+    if (i >= kMaxDenseClasses) {
+      set_error("run_dense_classes: too many classes for one launch");
+      return -1;
+    }
+    const direct_plan_s *dp = items[i].dp;
+    DenseClass &c = dc.c[i];
+    c.K = dp->d_K[dir];
+    c.off = items[i].off;
+    c.n = items[i].n;
+    c.nlines = items[i].nlines;
+    c.N = (int)dp->N;
+    c.NPP = dp->NPP;
+    c.bxs = (unsigned)(2 * ((c.nlines + kGemmLT - 1) / kGemmLT));
+    c.tiles = c.bxs * (unsigned)((dp->NPP + kGemmRT - 1) / kGemmRT);
+    total += c.tiles;
+  }
+  dc.count = count;
+  if (total == 0) return 0;
+  k_lines_gemm_multi<<<(unsigned)total, kGemmNT, gemm_shared_bytes(), st>>>(d_in, d_out, dc);
+  L2C_HIP(hipGetLastError(), -1);
+  return 0;
+}
+
 // Batched lines through a direct plan: dense MFMA products when the plan has
 // them and the batch is large enough, else one row per thread.
 static int launch_direct_lines(const direct_plan_s *dp, const double *d_in, double *d_out,
                                size_t dir, const LineMap &lm, size_t nlines, hipStream_t st) {
   const int N = (int)dp->N;
-  // the one-tile kernel (all rows of 32 lines per work-group) has the lower
-  // cost per line, the tiled one far more work-groups for a given call: above
-  // N = 32 the tiled kernel takes calls of up to kGemmFewLines lines (N = 256:
-  // 600 lines in 7 us against 18 us; 65,536 lines in 119 us against 90 us)
-  const bool tiled = N > kDenseMaxN || (dp->dense_tile == 2 && N > 32) ||
-                     (dp->dense_tile == 0 && N > 32 && nlines <= kGemmFewLines);
-  if (dp->d_K[dir] && nlines >= 4 && tiled) {
+  if (dense_tiled(dp, dir, nlines)) {
     const size_t nb = (nlines + kGemmLT - 1) / kGemmLT;
     const dim3 grid((unsigned)(2 * nb), (unsigned)((dp->NPP + kGemmRT - 1) / kGemmRT));
     if (lm.es == 1)

@@ -274,6 +274,16 @@ int run_lines(fmm_plan_s *p, const double *d_in, double *d_out, size_t dir, size
 int run_packed(fmm_plan_s *p, const direct_plan_s *dp, const double *d_in, double *d_out,
                size_t dir, size_t nlines, const size_t *d_off, const int *d_n, hipStream_t st);
 int reserve_lines(fmm_plan_s *p, size_t nlines);
+// packed lines of one direct plan for a fused dense launch
+struct DenseItem {
+  const direct_plan_s *dp;
+  const size_t *off;
+  const int *n;
+  size_t nlines;
+};
+bool dense_tiled(const direct_plan_s *dp, size_t dir, size_t nlines);
+int run_dense_classes(const DenseItem *items, int count, const double *d_in, double *d_out,
+                      size_t dir, hipStream_t st);
 int build_slices(fmm_plan_s *p, int dir);  // precision form 2 tables of one direction
 void drop_graphs(fmm_plan_s *p);  // synchronises the plan's stream first
 size_t flops_fmm(const fmm_plan_s *p, size_t dir);
