@@ -69,16 +69,16 @@ constexpr int kUpNT = 256;
 
 __host__ __device__ inline int up_xstride(int s) { return s | 1; }  // odd: conflict-free rows
 // LDS: input [2][16][xst], moments [16][2][M], parents [8+1][2][M], the
-// upward operators U(0), U(1) and the work-group's lu composite operators P,
-// rows padded to M+1
+// upward operators U(0), U(1) with rows padded to M+1 (the composite
+// operators P are read from L2 by the threads that apply them)
 size_t up_shared_bytes(const Geo &g) {
   const size_t M2 = 2 * g.M, Mp = g.M + 1;
-  return 8 * (2 * 16 * (size_t)up_xstride(g.s) + 16 * M2 + 9 * M2 +
-              (2 + (size_t)g.lu) * g.M * Mp);
+  return 8 * (2 * 16 * (size_t)up_xstride(g.s) + 16 * M2 + 9 * M2 + 2 * (size_t)g.M * Mp);
 }
 
 template <int MT>
-__global__ void __launch_bounds__(kUpNT) k_up(const double *__restrict__ in, LineMap lm, Geo g,
+__global__ void __launch_bounds__(kUpNT) __attribute__((amdgpu_waves_per_eu(4))) k_up(
+    const double *__restrict__ in, LineMap lm, Geo g,
                                               int dir, const double *__restrict__ T,
                                               const double *__restrict__ U,
                                               const double *__restrict__ P,
@@ -101,7 +101,6 @@ __global__ void __launch_bounds__(kUpNT) k_up(const double *__restrict__ in, Lin
   double *w0 = xs + 2 * 16 * xst;   // [16][2][M]
   double *w1 = w0 + 16 * M2;        // [8+1][2][M]
   double *Ul = w1 + 9 * M2;         // [2][M][M+1]: U(0), U(1)
-  double *Pl = Ul + 2 * M * Mp;     // [lu][M][M+1]: P(lu - l, j mod 2^(lu-l)), l < lu
 
   // ---- loads, all issued before any use: input, this thread's rows of the
   // upward operators (k = tid % M) and of its composite operator P
@@ -130,30 +129,22 @@ __global__ void __launch_bounds__(kUpNT) k_up(const double *__restrict__ in, Lin
   }
   const int ku = tid % M;
   const int np = g.lu * M2;
-  // the operators, once per work-group: U(0), U(1) and the lu composite
-  // operators, element e of the concatenation [U | P(lu) .. P(1)]; every load
-  // of the first batch is in flight before the first LDS store
-  const int nop = (2 + g.lu) * MM;
-  auto op_src = [&](int e) -> const double * {
-    if (e < 2 * MM) return U + e;
-    const int lev = e / MM - 2, r = e % MM;
-    const int k = g.lu - lev;
-    const size_t d = (size_t)j & (((size_t)1 << k) - 1);
-    return P + ((((size_t)1 << k) - 2) + d) * MM + r;
-  };
-  constexpr int QO = 12;
+  // U(0), U(1), once per work-group; every load of the batch is in flight
+  // before the first LDS store
+  const int nop = 2 * MM;
+  constexpr int QO = 3;
   double vo[QO];
 #pragma unroll
   for (int q = 0; q < QO; q++) {
     const int e = q * kUpNT + tid;
-    vo[q] = e < nop ? *op_src(e) : 0.0;
+    vo[q] = e < nop ? U[e] : 0.0;
   }
 #pragma unroll
   for (int q = 0; q < QO; q++) {
     const int e = q * kUpNT + tid;
-    if (e < nop) Ul[(e / M) * Mp + e % M] = vo[q];  // Pl follows Ul with the same row stride
+    if (e < nop) Ul[(e / M) * Mp + e % M] = vo[q];
   }
-  for (int e = QO * kUpNT + tid; e < nop; e += kUpNT) Ul[(e / M) * Mp + e % M] = *op_src(e);
+  for (int e = QO * kUpNT + tid; e < nop; e += kUpNT) Ul[(e / M) * Mp + e % M] = U[e];
   for (int e0 = 8 * kUpNT; e0 < nin; e0 += kUpNT) {  // rare: G*2s > 2048
     const int e = e0 + tid;
     if (e < nin) {
@@ -220,16 +211,8 @@ __global__ void __launch_bounds__(kUpNT) k_up(const double *__restrict__ in, Lin
   __syncthreads();
 
   // ---- step 2: parent = U0 child0 + U1 child1 inside the subtree; thread
-  // (k = tid % M) keeps its operator rows in registers
+  // (k = tid % M) reads its operator rows from LDS
   const int npair = kUpNT / M;  // (chunk, parity) pairs handled per sweep
-  double u0r[MT > 0 ? MT : 1], u1r[MT > 0 ? MT : 1];  // this thread's operator rows
-  if constexpr (MT > 0) {
-#pragma unroll
-    for (int l = 0; l < MT; l++) {
-      u0r[l] = Ul[ku * Mp + l];
-      u1r[l] = Ul[(M + ku) * Mp + l];
-    }
-  }
   int n = G;
   for (int lev = L - 2; lev >= g.lu; lev--) {
     n >>= 1;
@@ -243,11 +226,11 @@ __global__ void __launch_bounds__(kUpNT) k_up(const double *__restrict__ in, Lin
         if constexpr (MT > 0) {
 #pragma unroll
           for (int l = 0; l < MT; l += 2) {
-            a0 += u0r[l] * x0[l];
-            a1 += u1r[l] * x1[l];
+            a0 += u0[l] * x0[l];
+            a1 += u1[l] * x1[l];
             if (l + 1 < MT) {
-              a2 += u0r[l + 1] * x0[l + 1];
-              a3 += u1r[l + 1] * x1[l + 1];
+              a2 += u0[l + 1] * x0[l + 1];
+              a3 += u1[l + 1] * x1[l + 1];
             }
           }
         } else {
@@ -272,7 +255,9 @@ __global__ void __launch_bounds__(kUpNT) k_up(const double *__restrict__ in, Lin
   for (int task = tid; task < np; task += kUpNT) {
     const int lev = task / M2, o = task % M2, sg = o / M, i = o % M;
     const double *x = w0 + sg * M;
-    const double *prow = Pl + (lev * M + i) * Mp;
+    const int k = g.lu - lev;
+    const size_t d = (size_t)j & (((size_t)1 << k) - 1);
+    const double *prow = P + ((((size_t)1 << k) - 2) + d) * MM + (size_t)i * M;
     double acc = 0;
 #pragma unroll
     for (int l = 0; l < (MT > 0 ? MT : 64); l++) {
@@ -1948,6 +1933,16 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) k
     vd[q] = Dg[e < 2 * MM ? e : 0];
   }
   __builtin_amdgcn_sched_barrier(0);
+  // the band sums this work-group adds to, in flight through the tree pass
+  // (nr <= 2 G s <= 8 NT)
+  constexpr int QB = 8;
+  double vb[QB];
+#pragma unroll
+  for (int q = 0; q < QB; q++) {
+    const int e = q * NT + tid;
+    vb[q] = dst[(row0 + (e < nr ? e : 0)) * lm.es];
+  }
+  __builtin_amdgcn_sched_barrier(0);
 #pragma unroll
   for (int q = 0; q < QM; q++) {
     const int e = q * NT + tid;
@@ -2005,9 +2000,12 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) k
   }
   lds_sync();
   // ---- the band sums plus the far field, scaled
-  for (int e = tid; e < nr; e += NT) {
+#pragma unroll
+  for (int q = 0; q < QB; q++) {
+    const int e = q * NT + tid;
+    if (e >= nr) continue;
     const size_t i = row0 + e;
-    double v = dst[i * lm.es] + ob[slot(e)];
+    double v = vb[q] + ob[slot(e)];
     if (DIR == 0) {
       v *= k2Pi;
       if (i == 0) v *= 0.5;
@@ -2575,9 +2573,15 @@ static void launch_far(fmm_plan_s *p, double *d_out, size_t dir, const LineMap &
   prof_end(p->prof, st);
 }
 
-// Whether this call takes the split downward pass (k_band + k_far).
+// Whether this call takes the split downward pass (k_band + k_far): always
+// (down_split 1), never (0), or (2, the default) for calls of several lines
+// of deep trees (L >= 10, N >= 2^18 at s = 64), where its leaner kernels
+// overlap better: 8 lines of 2^20 in 293 us against 315 us fused, 32 lines
+// in 1017 against 1116 us; one line 58 against 56 us, 32 lines of 2^16 92
+// against 89 us.
 static bool use_split(const fmm_plan_s *p, size_t nlines) {
-  return p->geo.fast && (p->down_split == 1 || (p->down_split == 2 && nlines > 1));
+  return p->geo.fast &&
+         (p->down_split == 1 || (p->down_split == 2 && nlines > 1 && p->geo.L >= 10));
 }
 
 // One execute (or a batch of lines): upward pass, M2L, downward pass, all on st.

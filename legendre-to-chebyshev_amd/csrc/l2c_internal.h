@@ -215,8 +215,8 @@ struct fmm_plan_s {
   // (0: never); L2C_M2L_MMA sets it at plan creation
   int m2l_mma_lines = 0;
   // downward pass of s = 16/32/64 plans: 0 the fused k_down8, 1 k_band +
-  // k_far, 2 k_band + k_far for calls of more than one line
-  int down_split = 0;
+  // k_far, 2 (default) k_band + k_far for calls of several lines with L >= 10
+  int down_split = 2;
   l2c::Geo geo{};
   l2c::DevTables t;
   direct_plan_s *dplan = nullptr;  // L < 1: direct path only
