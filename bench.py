@@ -4,7 +4,7 @@
 A step is one Legendre->Chebyshev transform of one synthetic coefficient
 vector (U(-1,1), SplitMix64 seed 1) that is already resident in HBM.  Warmup
 steps are untimed; the timed region is bracketed by device synchronisation.
-Every execute kernel is timed live with HIP events on its launch stream; the
+Every execute kernel is timed live with HIP events on its own dispatch; the
 dominant kernel's average duration gives the roofline figure.  The result is
 checked against the oracle (CPU restatement of the reference's execute) and
 against extended-precision direct sums on sampled rows.  The oracle, run on
@@ -683,9 +683,9 @@ def main():
     elapsed = time.perf_counter() - t0
     value = args.steps / elapsed
     ms_per_step = 1e3 * elapsed / args.steps
-    # kernel durations: a separate pass of the same steps with HIP events on
-    # the launch stream around every kernel (kept out of the timed loop,
-    # whose throughput the events would lower)
+    # kernel durations: a separate pass of the same steps in which every
+    # kernel launch carries dispatch start/stop HIP events (kept out of the
+    # timed loop, whose throughput the event handling would lower)
     prof_steps = max(20, args.steps // 4) if args.prof_every > 0 else 0
     kernels = {}
     if prof_steps:
@@ -714,17 +714,20 @@ def main():
     tol = 1e-14 * math.log2(N)
     log(f"[bench] rel err vs oracle FMM {err_fmm:.3e}, vs double-double direct (rows) {err_direct:.3e}")
 
-    # roofline of the dominant (longest) kernel
-    dom = max(kernels, key=lambda k: kernels[k]["avg_us"])
-    avg_us = kernels[dom]["avg_us"]
-    alg = kernel_bytes(dom, info, N)
-    achieved = alg / (avg_us * 1e-6) / 1e9
+    # roofline of the dominant (longest) kernel (none when the timing pass
+    # is off, as in the counter runs)
     workload = workload_name(N, maxs, M, d)
     assert f"_s{info['s']}_" in workload
-    roofline = {"bound": "hbm", "kernel": dom, "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
-                "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
-                "traffic": pmc_traffic(workload, dom), "algorithmic_bytes": alg,
-                "avg_launch_us": round(avg_us, 3)}
+    roofline, dom, alg = None, None, None
+    if kernels:
+        dom = max(kernels, key=lambda k: kernels[k]["avg_us"])
+        avg_us = kernels[dom]["avg_us"]
+        alg = kernel_bytes(dom, info, N)
+        achieved = alg / (avg_us * 1e-6) / 1e9
+        roofline = {"bound": "hbm", "kernel": dom, "achieved": round(achieved, 1),
+                    "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
+                    "traffic": pmc_traffic(workload, dom), "algorithmic_bytes": alg,
+                    "avg_launch_us": round(avg_us, 3)}
 
     # cold cache: before each transform a 256 MiB device copy (512 MiB of
     # traffic, twice the 256 MB last-level cache) evicts the plan's tables,
@@ -748,7 +751,7 @@ def main():
         cold = {"kernels_us": ck, "sum_us": round(csum, 3), "warm_sum_us": round(wsum, 3),
                 "transforms_s_from_kernel_sum": round(1e6 / csum, 1), "steps": args.cold_steps,
                 "flush": "256 MiB device-to-device copy before each transform"}
-        cd = cold["kernels_us"].get(dom)
+        cd = cold["kernels_us"].get(dom) if dom else None
         if cd:
             cold["dominant_achieved_GBps"] = round(alg / (cd * 1e-6) / 1e9, 1)
         gc = dout.download()

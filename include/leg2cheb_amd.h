@@ -174,7 +174,7 @@ int l2c_execute_device(fmm_plan *plan, const double *d_in, double *d_out,
  * direction, stride) and replays it on later calls with the same arguments
  * (the launches are identical; replay saves host launch work).  Up to eight
  * argument sets are kept; further ones launch directly, as do the calls that
- * l2c_profile times (their kernels run between the timing events).
+ * l2c_profile times (their launches carry the timing events).
  * enable = 0 releases the graphs (after synchronising the plan's stream).
  * Returns 0 or -1. */
 int l2c_plan_graph(fmm_plan *plan, int enable);
@@ -310,13 +310,16 @@ int l2c_execute_dct_lines_device(l2c_dct_plan *p, const double *d_in, double *d_
                                  size_t nlines, size_t n0, size_t line_stride0,
                                  size_t line_stride1, size_t elem_stride, void *stream);
 
-/* Per-kernel timing with HIP events recorded on the launch stream around
- * the execute kernels.  l2c_profile(plan, n) resets the sums and, for n >= 1,
- * times the kernels of every n-th call (execute, directM, lines); n = 0 turns
- * it off.  Events between kernels cost several microseconds of GPU time each,
- * so a period n > 1 keeps the measured throughput close to the unprofiled
- * one.  l2c_profile_read fills ms[8] (summed milliseconds) and counts[8]
- * (timed launches) indexed 0 k_up, 1 k_m2l, 2 k_down, 3 k_direct, 4-7 unused. */
+/* Per-kernel timing of the execute kernels with HIP events.  Each timed
+ * launch carries a start/stop event pair that the runtime fills from its own
+ * dispatch (hipExtLaunchKernel), so a launch's time is its kernel's
+ * execution, without launch gaps.  l2c_profile(plan, n) resets the sums and,
+ * for n >= 1, times the kernels of every n-th call (execute, directM, lines);
+ * n = 0 turns it off.  l2c_profile_read fills ms[8] (summed milliseconds)
+ * and counts[8] (timed calls per slot; a slot that runs two kernels in one
+ * call, as the M2L of precision form 2 does, sums both and counts one)
+ * indexed 0 k_up, 1 k_m2l, 2 k_down, 3 k_direct (and the dense line
+ * products), 4 k_far, 5 k_band, 6-7 unused. */
 int l2c_profile(fmm_plan *plan, int enable);
 int l2c_profile_read(fmm_plan *plan, double *ms, size_t *counts);
 

@@ -23,32 +23,43 @@ void prof_step(Prof *pf) {
 }
 
 void prof_begin(Prof *pf, int id, hipStream_t st) {
+  (void)st;
   if (!pf || !pf->on || !pf->cur) return;
-  if (pf->used + 2 > pf->pool.size()) {
-    for (int i = 0; i < 256; i++) {
-      hipEvent_t e;
-      if (hipEventCreate(&e) != hipSuccess) return;
-      pf->pool.push_back(e);
-    }
-  }
-  Prof::Rec r{id, pf->pool[pf->used], pf->pool[pf->used + 1]};
-  pf->used += 2;
-  (void)hipEventRecord(r.a, st);
-  pf->recs.push_back(r);
+  pf->open = id;
+  pf->brackets++;
 }
 
 void prof_end(Prof *pf, hipStream_t st) {
-  if (!pf || !pf->on || !pf->cur || pf->recs.empty()) return;
-  (void)hipEventRecord(pf->recs.back().b, st);
+  (void)st;
+  if (pf) pf->open = -1;
+}
+
+bool prof_events(Prof *pf, hipEvent_t *a, hipEvent_t *b) {
+  if (!pf || !pf->on || !pf->cur || pf->open < 0) return false;
+  if (pf->used + 2 > pf->pool.size()) {
+    for (int i = 0; i < 256; i++) {
+      hipEvent_t e;
+      if (hipEventCreate(&e) != hipSuccess) return false;
+      pf->pool.push_back(e);
+    }
+  }
+  *a = pf->pool[pf->used];
+  *b = pf->pool[pf->used + 1];
+  pf->used += 2;
+  pf->recs.push_back(Prof::Rec{pf->open, pf->brackets, *a, *b});
+  return true;
 }
 
 static void prof_harvest(Prof *pf) {
+  size_t last[K_COUNT];
+  for (int i = 0; i < K_COUNT; i++) last[i] = 0;
   for (auto &r : pf->recs) {
     (void)hipEventSynchronize(r.b);
     float ms = 0;
     if (hipEventElapsedTime(&ms, r.a, r.b) == hipSuccess) {
       pf->ms[r.id] += ms;
-      pf->cnt[r.id]++;
+      if (last[r.id] != r.bracket) pf->cnt[r.id]++;
+      last[r.id] = r.bracket;
     }
   }
   pf->recs.clear();
@@ -117,8 +128,8 @@ static bool is_device_ptr(const void *ptr) {
 
 // One execute through a cached HIP graph of its launch sequence: captured on
 // the plan's stream by the first call with this key, replayed on st after.
-// A call that l2c_profile times runs its launches directly between its
-// events (timing events stay out of the graphs), as does a call whose key
+// A call that l2c_profile times launches directly (its launches carry the
+// timing events, which stay out of the graphs), as does a call whose key
 // finds the cache full.  Returns 0 or -1.
 static int run_fmm_graph(fmm_plan_s *p, const double *d_in, double *d_out, size_t dir,
                          size_t stride, hipStream_t st) {

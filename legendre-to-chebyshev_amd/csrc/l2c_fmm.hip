@@ -2416,8 +2416,8 @@ static void launch_direct(const direct_plan_s *dp, const double *d_in, double *d
   const int N = (int)dp->N;
   const unsigned gy = (unsigned)std::min<size_t>(nlines, dp->lines_y);
   const unsigned gz = (unsigned)((nlines + gy - 1) / gy);
-  k_direct<<<dim3((N + 255) / 256, gy, gz), 256, 0, st>>>(d_in, d_out, lm, N, (int)dir, dp->d_a,
-                                                          dp->d_an, dp->d_dn, nlines);
+  launch_k(dp->prof, k_direct, dim3((N + 255) / 256, gy, gz), 256, 0, st, d_in, d_out, lm, N,
+      (int)dir, dp->d_a, dp->d_an, dp->d_dn, nlines);
 }
 
 // Whether a call of nlines lines through direct plan dp takes the tiled dense
@@ -2471,19 +2471,19 @@ static int launch_direct_lines(const direct_plan_s *dp, const double *d_in, doub
     const size_t nb = (nlines + kGemmLT - 1) / kGemmLT;
     const dim3 grid((unsigned)(2 * nb), (unsigned)((dp->NPP + kGemmRT - 1) / kGemmRT));
     if (lm.es == 1)
-      k_lines_gemm<true><<<grid, kGemmNT, gemm_shared_bytes(), st>>>(d_in, d_out, lm, nlines, N,
-                                                                      dp->NPP, dp->d_K[dir]);
+      launch_k(dp->prof, k_lines_gemm<true>, grid, kGemmNT, gemm_shared_bytes(), st, d_in, d_out,
+          lm, nlines, N, dp->NPP, dp->d_K[dir]);
     else
-      k_lines_gemm<false><<<grid, kGemmNT, gemm_shared_bytes(), st>>>(d_in, d_out, lm, nlines, N,
-                                                                       dp->NPP, dp->d_K[dir]);
+      launch_k(dp->prof, k_lines_gemm<false>, grid, kGemmNT, gemm_shared_bytes(), st, d_in, d_out,
+          lm, nlines, N, dp->NPP, dp->d_K[dir]);
   } else if (dp->d_K[dir] && nlines >= 4) {
     const size_t nb = (nlines + kDenseLT - 1) / kDenseLT;
     if (lm.es == 1)
-      k_lines_dense<true><<<dim3((unsigned)nb), kDenseNT, dense_shared_bytes(dp->NPP), st>>>(
-          d_in, d_out, lm, nlines, N, dp->NPP, dp->d_K[dir]);
+      launch_k(dp->prof, k_lines_dense<true>, dim3((unsigned)nb), kDenseNT,
+          dense_shared_bytes(dp->NPP), st, d_in, d_out, lm, nlines, N, dp->NPP, dp->d_K[dir]);
     else
-      k_lines_dense<false><<<dim3((unsigned)nb), kDenseNT, dense_shared_bytes(dp->NPP), st>>>(
-          d_in, d_out, lm, nlines, N, dp->NPP, dp->d_K[dir]);
+      launch_k(dp->prof, k_lines_dense<false>, dim3((unsigned)nb), kDenseNT,
+          dense_shared_bytes(dp->NPP), st, d_in, d_out, lm, nlines, N, dp->NPP, dp->d_K[dir]);
   } else {
     launch_direct(dp, d_in, d_out, dir, lm, nlines, st);
   }
@@ -2525,14 +2525,14 @@ static void launch_down8(fmm_plan_s *p, const double *d_in, double *d_out, size_
   const double *htab = p->t.head[dir];
   prof_begin(p->prof, K_DOWN_BAND, st);
   if (dir == L2C)
-    k_down8<0, 0, MT><<<gd, nt, shm, st>>>(d_in, d_out, lm, g, mode, ws, wsstride, p->t.D,
-                                           p->t.Q, p->t.T, tab, htab);
+    launch_k(p->prof, k_down8<0, 0, MT>, gd, nt, shm, st, d_in, d_out, lm, g, mode, ws, wsstride,
+        p->t.D, p->t.Q, p->t.T, tab, htab);
   else if (p->precompute)
-    k_down8<1, 1, MT><<<gd, nt, shm, st>>>(d_in, d_out, lm, g, mode, ws, wsstride, p->t.D,
-                                           p->t.Q, p->t.T, tab, htab);
+    launch_k(p->prof, k_down8<1, 1, MT>, gd, nt, shm, st, d_in, d_out, lm, g, mode, ws, wsstride,
+        p->t.D, p->t.Q, p->t.T, tab, htab);
   else
-    k_down8<1, 0, MT><<<gd, nt, shm, st>>>(d_in, d_out, lm, g, mode, ws, wsstride, p->t.D,
-                                           p->t.Q, p->t.T, tab, htab);
+    launch_k(p->prof, k_down8<1, 0, MT>, gd, nt, shm, st, d_in, d_out, lm, g, mode, ws, wsstride,
+        p->t.D, p->t.Q, p->t.T, tab, htab);
   prof_end(p->prof, st);
 }
 
@@ -2549,11 +2549,11 @@ static void launch_band(fmm_plan_s *p, const double *d_in, double *d_out, size_t
   const double *htab = p->t.head[dir];
   prof_begin(p->prof, K_BAND, st);
   if (dir == L2C)
-    k_band<0, 0><<<gd, nt, shm, st>>>(d_in, d_out, lm, g, scale, tab, htab);
+    launch_k(p->prof, k_band<0, 0>, gd, nt, shm, st, d_in, d_out, lm, g, scale, tab, htab);
   else if (p->precompute)
-    k_band<1, 1><<<gd, nt, shm, st>>>(d_in, d_out, lm, g, scale, tab, htab);
+    launch_k(p->prof, k_band<1, 1>, gd, nt, shm, st, d_in, d_out, lm, g, scale, tab, htab);
   else
-    k_band<1, 0><<<gd, nt, shm, st>>>(d_in, d_out, lm, g, scale, tab, htab);
+    launch_k(p->prof, k_band<1, 0>, gd, nt, shm, st, d_in, d_out, lm, g, scale, tab, htab);
   prof_end(p->prof, st);
 }
 
@@ -2565,11 +2565,11 @@ static void launch_far(fmm_plan_s *p, double *d_out, size_t dir, const LineMap &
   const int nt = 64 * (g.s >> 4);
   prof_begin(p->prof, K_FAR, st);
   if (dir == L2C)
-    k_far<0, MT><<<gd, nt, far_shared_bytes(g), st>>>(d_out, lm, g, ws, ws_doubles(g), p->t.D,
-                                                      p->t.Q, p->t.T);
+    launch_k(p->prof, k_far<0, MT>, gd, nt, far_shared_bytes(g), st, d_out, lm, g, ws,
+        ws_doubles(g), p->t.D, p->t.Q, p->t.T);
   else
-    k_far<1, MT><<<gd, nt, far_shared_bytes(g), st>>>(d_out, lm, g, ws, ws_doubles(g), p->t.D,
-                                                      p->t.Q, p->t.T);
+    launch_k(p->prof, k_far<1, MT>, gd, nt, far_shared_bytes(g), st, d_out, lm, g, ws,
+        ws_doubles(g), p->t.D, p->t.Q, p->t.T);
   prof_end(p->prof, st);
 }
 
@@ -2603,8 +2603,8 @@ static int launch_fmm_m(fmm_plan_s *p, const double *d_in, double *d_out, size_t
   if (!band_only) {
     dim3 gu((unsigned)((size_t)4 << g.lu), (unsigned)nlines);
     prof_begin(p->prof, K_UP_SUBTREE, st);
-    k_up<MT><<<gu, kUpNT, up_shared_bytes(g), st>>>(d_in, lm, g, (int)dir, p->t.T, p->t.U,
-                                                    p->t.P, ws, wsstride);
+    launch_k(p->prof, k_up<MT>, gu, kUpNT, up_shared_bytes(g), st, d_in, lm, g, (int)dir, p->t.T,
+        p->t.U, p->t.P, ws, wsstride);
     prof_end(p->prof, st);
     const int ntop = (int)total_blocks(g.lu), ntot = (int)total_blocks(g.L);
     const int bpw = m2l_bpw(g);
@@ -2621,14 +2621,14 @@ static int launch_fmm_m(fmm_plan_s *p, const double *d_in, double *d_out, size_t
     const bool mma = MT == 18 && (p->precision == 2 ||
                                   (p->m2l_mma_lines > 0 && nlines >= (size_t)p->m2l_mma_lines));
     if (!mma) {
-      k_m2l<MT><<<gm, kM2lNT, m2l_shared_bytes(g), st>>>(g, p->t.A[dir], ws, wsstride, bpw,
-                                                        nlines, lpb, sv);
+      launch_k(p->prof, k_m2l<MT>, gm, kM2lNT, m2l_shared_bytes(g), st, g, p->t.A[dir], ws,
+          wsstride, bpw, nlines, lpb, sv);
     } else {
       // the blocks above lu (multipoles summed from the k_up contributions)
       // as before; the rest on MFMA
       if (ntop > 0)
-        k_m2l<MT><<<dim3((unsigned)ntop, gm.y), kM2lNT, m2l_shared_bytes(g), st>>>(
-            g, p->t.A[dir], ws, wsstride, bpw, nlines, lpb, sv);
+        launch_k(p->prof, k_m2l<MT>, dim3((unsigned)ntop, gm.y), kM2lNT, m2l_shared_bytes(g), st, g,
+            p->t.A[dir], ws, wsstride, bpw, nlines, lpb, sv);
       const int nblk = ntot - ntop;
       const unsigned mx = (unsigned)((nblk + 3) / 4);
       const size_t ngroups = (nlines + 7) / 8;
@@ -2638,11 +2638,11 @@ static int launch_fmm_m(fmm_plan_s *p, const double *d_in, double *d_out, size_t
       const dim3 gmm(mx, (unsigned)((ngroups + gpw - 1) / gpw));
       if (p->precision == 2) {
         const DevTables::Slices &sl = p->t.sl[dir];
-        k_m2l_mma<2><<<gmm, 256, 0, st>>>(g, p->t.A[dir], (const bf16x8 *)sl.q, sl.sc, ws,
-                                          wsstride, nlines, (int)gpw);
+        launch_k(p->prof, k_m2l_mma<2>, gmm, 256, 0, st, g, p->t.A[dir], (const bf16x8 *)sl.q,
+            sl.sc, ws, wsstride, nlines, (int)gpw);
       } else {
-        k_m2l_mma<0><<<gmm, 256, 4 * 3 * 18 * 19 * sizeof(double), st>>>(
-            g, p->t.A[dir], nullptr, nullptr, ws, wsstride, nlines, (int)gpw);
+        launch_k(p->prof, k_m2l_mma<0>, gmm, 256, 4 * 3 * 18 * 19 * sizeof(double), st, g,
+            p->t.A[dir], nullptr, nullptr, ws, wsstride, nlines, (int)gpw);
       }
     }
     prof_end(p->prof, st);
@@ -2661,14 +2661,14 @@ static int launch_fmm_m(fmm_plan_s *p, const double *d_in, double *d_out, size_t
   const double *htab = p->t.head[dir];
   prof_begin(p->prof, K_DOWN_BAND, st);
   if (dir == L2C)
-    k_down<0, 0, MT><<<gd, kDownNT, shm, st>>>(d_in, d_out, lm, g, band_only, ws, wsstride,
-                                               p->t.D, p->t.Q, p->t.T, tab, htab);
+    launch_k(p->prof, k_down<0, 0, MT>, gd, kDownNT, shm, st, d_in, d_out, lm, g, band_only, ws,
+        wsstride, p->t.D, p->t.Q, p->t.T, tab, htab);
   else if (p->precompute)
-    k_down<1, 1, MT><<<gd, kDownNT, shm, st>>>(d_in, d_out, lm, g, band_only, ws, wsstride,
-                                               p->t.D, p->t.Q, p->t.T, tab, htab);
+    launch_k(p->prof, k_down<1, 1, MT>, gd, kDownNT, shm, st, d_in, d_out, lm, g, band_only, ws,
+        wsstride, p->t.D, p->t.Q, p->t.T, tab, htab);
   else
-    k_down<1, 0, MT><<<gd, kDownNT, shm, st>>>(d_in, d_out, lm, g, band_only, ws, wsstride,
-                                               p->t.D, p->t.Q, p->t.T, tab, htab);
+    launch_k(p->prof, k_down<1, 0, MT>, gd, kDownNT, shm, st, d_in, d_out, lm, g, band_only, ws,
+        wsstride, p->t.D, p->t.Q, p->t.T, tab, htab);
   prof_end(p->prof, st);
   return 0;
 }

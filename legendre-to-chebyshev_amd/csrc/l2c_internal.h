@@ -3,6 +3,7 @@
 #pragma once
 
 #include <hip/hip_runtime.h>
+#include <hip/hip_ext.h>
 
 #include <cstddef>
 #include <cstdint>
@@ -168,8 +169,11 @@ struct Prof {
   int on = 0;       // timing period in calls (0: off)
   size_t calls = 0;  // calls since enabled
   int cur = 0;      // the current call is timed
+  int open = -1;     // kernel slot of the open bracket (-1: none)
+  size_t brackets = 0;
   struct Rec {
     int id;
+    size_t bracket;
     hipEvent_t a, b;
   };
   std::vector<Rec> recs;
@@ -179,8 +183,28 @@ struct Prof {
   size_t cnt[K_COUNT] = {0};
 };
 void prof_step(Prof *pf);  // at the start of each call
+// A bracket around the launches of one kernel slot: every launch inside it
+// adds its execution time to the slot, and the bracket counts one launch.
 void prof_begin(Prof *pf, int id, hipStream_t st);
 void prof_end(Prof *pf, hipStream_t st);
+// A start/stop event pair for the next launch when the current call is
+// timed and a bracket is open; false otherwise.
+bool prof_events(Prof *pf, hipEvent_t *a, hipEvent_t *b);
+
+// Launch kern on st.  Inside an open bracket of a timed call the launch
+// carries the start/stop events that the runtime fills from the dispatch
+// itself (hipExtLaunchKernel), so the recorded time is the kernel's
+// execution alone, without the dispatch gaps that events recorded between
+// launches would add.
+template <typename... KArgs, typename... Args>
+inline void launch_k(Prof *pf, void (*kern)(KArgs...), dim3 grid, dim3 block, size_t shm,
+                     hipStream_t st, Args... args) {
+  hipEvent_t a, b;
+  if (prof_events(pf, &a, &b))
+    hipExtLaunchKernelGGL(kern, grid, block, (std::uint32_t)shm, st, a, b, 0u, args...);
+  else
+    kern<<<grid, block, shm, st>>>(args...);
+}
 }  // namespace l2c
 
 struct direct_plan_s {
