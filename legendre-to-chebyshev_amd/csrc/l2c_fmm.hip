@@ -948,12 +948,13 @@ __device__ __forceinline__ const double *qrow_ptr(const double *Qg, const Geo &g
 // run on MFMA with D(p) from the LDS copy Dl ([p][k][M + 1], padded rows);
 // else on VALU with D(p) row tid % M in d0r/d1r (MT > 0) or from Dg.  Every
 // thread of the work-group must call this.
-template <int MT, bool DLDS>
+template <int MT, bool DLDS, class Sync>
 __device__ __forceinline__ void subtree_locals(const Geo &g, int Rr, int tid, int NT,
                                                const double *mres, double *qp, double *cres,
                                                const double *qr, const double *d0r,
                                                const double *d1r, const double *Qg,
-                                               const double *Dg, const double *Dl) {
+                                               const double *Dg, const double *Dl,
+                                               Sync &&sync) {
   const int M = MT > 0 ? MT : g.M, M2 = 2 * M, MM = M * M;
   const int nq = g.ld * M;
   int task0 = tid;
@@ -982,7 +983,7 @@ __device__ __forceinline__ void subtree_locals(const Geo &g, int Rr, int tid, in
     qp[lev * M2 + k] = o0;
     qp[lev * M2 + M + k] = o1;
   }
-  lds_sync();
+  sync();
   for (int o = tid; o < M2; o += NT) {
     // the chain terms in level order; four reads in flight per step
     double v = mres[g.ld * M2 + o];
@@ -998,7 +999,7 @@ __device__ __forceinline__ void subtree_locals(const Geo &g, int Rr, int tid, in
     for (; lev < g.ld; lev++) v += qp[lev * M2 + o];
     cres[o] = v;
   }
-  lds_sync();
+  sync();
   if (DLDS) {
     // levels ld+1..ld+gd on v_mfma_f64_16x16x4f64: children of parity p are
     // C (M x n) = D(p) (M x M) C_parents (M x n), n = 2^lam children x 2
@@ -1044,7 +1045,7 @@ __device__ __forceinline__ void subtree_locals(const Geo &g, int Rr, int tid, in
           }
         }
       }
-      lds_sync();
+      sync();
     }
     return;
   }
@@ -1065,7 +1066,7 @@ __device__ __forceinline__ void subtree_locals(const Geo &g, int Rr, int tid, in
         }
         cres[(b0 + ci) * M2 + oo] = v;
       }
-      lds_sync();
+      sync();
     }
     return;
   }
@@ -1096,7 +1097,7 @@ __device__ __forceinline__ void subtree_locals(const Geo &g, int Rr, int tid, in
         cres[(b0 + ci) * M2 + oo] = v;
       }
     }
-    lds_sync();
+    sync();
   }
 }
 
@@ -1205,7 +1206,8 @@ __global__ void __launch_bounds__(kDownNT) __attribute__((amdgpu_waves_per_eu(2)
 
   // ---- phase 2: local expansions of the subtree
   if (!band_only)
-    subtree_locals<MT, false>(g, Rr, tid, NT, mres, qp, cres, qr, d0r, d1r, Qg, Dg, nullptr);
+    subtree_locals<MT, false>(g, Rr, tid, NT, mres, qp, cres, qr, d0r, d1r, Qg, Dg, nullptr,
+                             LdsSync{});
 
   // ---- phase 3: band + leaf evaluation.  Task (chunk, group gam): rows
   // gam*4.. of parity 0, then rows (ngrp-1-gam)*4.. of parity 1, so every
@@ -1547,7 +1549,8 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) k
 
   // ---- phase 2: local expansions of the subtree
   if (do_far)
-    subtree_locals<MT, true>(g, Rr, tid, NT, mres, qp, cres, qr, nullptr, nullptr, Qg, Dg, dl);
+    subtree_locals<MT, true>(g, Rr, tid, NT, mres, qp, cres, qr, nullptr, nullptr, Qg, Dg, dl,
+                            LdsSync{});
 
   // A operand (T) of this wave's two far-field tiles, in flight during the band
   const int ntm = s >> 4, ksteps = (M + 3) >> 2;
@@ -1700,6 +1703,313 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) k
   for (int e = tid; e < nr; e += NT) {
     const size_t i = row0 + e;
     double v = ob[slot(e)] + ob2[slot(e)];
+    if (DIR == 0) {
+      v *= k2Pi;
+      if (i == 0) v *= 0.5;
+    } else {
+      v *= (double)i + 0.5;
+    }
+    dst[i * lm.es] = v;
+  }
+}
+
+// ---------------------------------------------------------------------------
+// k_down8r: k_down8's work (downward pass, near-field band, leaf evaluation;
+// src/C/leg2cheb.c:1437-1483 and the band of :236-535) with wave roles.  The
+// work-group has 2 s/16 waves: the first s/16 (the tree role) load the M2L
+// vectors and operators, run the ancestor chain and the subtree's local
+// expansions and evaluate them at the leaf nodes on MFMA (as k_far); the
+// others (the band role) load the input and Lambda windows and sum the band,
+// one row pair of 8 rows over all its columns per wave.  The roles share no
+// LDS and synchronise among themselves with RoleSync; the one hardware
+// barrier between them is the join before the scaled sum is stored.  So the
+// band's FP64 VALU work overlaps the tree role's loads, LDS round trips and
+// dependent MFMA steps inside each work-group, at 128 VGPRs and 78.5 KB of
+// LDS (two work-groups, 16 waves, per CU at s = 64).
+struct Down8rLds {
+  int xs, nxs, lw, LW, hd, cres, dl, mres, qp, obf, ctr, total;
+};
+__host__ __device__ inline Down8rLds down8r_lds(const Geo &g) {
+  const Down8Lds d8 = down8_lds(g);
+  const int G = 1 << g.gd, s = g.s, M2 = 2 * g.M;
+  Down8rLds d;
+  d.xs = d8.xs;
+  d.nxs = d8.nxs;
+  d.lw = d8.lw;
+  d.LW = d8.LW;
+  d.hd = d8.hd;
+  d.cres = d.hd + 2 * s + 16;
+  d.dl = d.cres + (2 * G - 1) * M2;  // D(0), D(1) with rows padded to M + 1
+  d.mres = d.dl + g.M * M2 + M2;
+  d.qp = d.mres + (g.ld + 2 * G - 1) * M2;
+  const int tree_end = d.qp + std::max(g.ld, 1) * M2;
+  d.obf = d.dl;  // far-field staging (2 G s + G) over the tree's dead operands
+  d.ctr = std::max(tree_end, d.obf + 2 * G * s + G);
+  d.total = d.ctr + 1;  // the two roles' barrier counters
+  return d;  // the band staging (2 G s + G) reuses the x windows
+}
+size_t down8r_shared_bytes(const Geo &g) { return 8 * (size_t)down8r_lds(g).total; }
+
+template <int DIR, int PRE, int MT>
+__global__ void __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(4))) k_down8r(
+    const double *__restrict__ in, double *__restrict__ out, LineMap lm, Geo g,
+    const double *__restrict__ wsall, size_t wsstride, const double *__restrict__ Dg,
+    const double *__restrict__ Qg, const double *__restrict__ Tg, const double *__restrict__ tab,
+    const double *__restrict__ head, int shift) {
+  extern __shared__ double sm[];
+  const int M = MT > 0 ? MT : g.M;
+  const int s = g.s, N = g.N, M2 = 2 * M, MM = M * M;
+  const int Nl = line_len(lm, blockIdx.y, N);
+  const int G = 1 << g.gd, sh = 31 - __builtin_clz(s);
+  const int nw = s >> 4, NT = 64 * nw;  // waves and threads of each role
+  const int Rr = blockIdx.x;
+  const int lane = threadIdx.x & 63;
+  const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  const bool tree = wave < nw;
+  const int tid = tree ? (int)threadIdx.x : (int)threadIdx.x - NT;  // index in the role
+  const size_t row0 = (size_t)Rr * G * 2 * s;
+  if (row0 >= (size_t)Nl) return;  // the whole work-group
+  const int nr = (int)std::min((size_t)G * 2 * s, (size_t)Nl - row0);
+  const size_t hc0 = (size_t)Rr * G * s;
+  const size_t off = line_off(lm, blockIdx.y);
+  const double *src = in + off;
+  double *dst = out + off;
+  const Down8rLds ly = down8r_lds(g);
+  double *xs = sm + ly.xs, *lw = sm + ly.lw, *hd = sm + ly.hd;
+  double *cres = sm + ly.cres, *dl = sm + ly.dl, *mres = sm + ly.mres, *qp = sm + ly.qp;
+  double *ob = xs, *obf = sm + ly.obf;
+  int *ctr = reinterpret_cast<int *>(sm + ly.ctr);
+  auto slot = [&](int e) { return e + (e >> (sh + 1)); };
+  // band lanes (both roles): chunk gl = lane % 16, parity sg = lane / 16 % 2,
+  // del = lane / 32; row group gam = 2 p + del of pair p holds rows 8 gam ..
+  // 8 gam + 7 of its chunk.  Computed where used, so that nothing of it is
+  // live through the tree role's phases.
+  // columns [t0, t1) of pair p's row group (t1 < 0: to its end) added to acc
+  auto band = [&](int p, int t0, int t1, double acc[8]) {
+    const int gl = lane & 15, sg = (lane >> 4) & 1, gam = 2 * p + (lane >> 5);
+    const int cnt = (N - sg + 1) / 2;
+    const size_t beta = (size_t)Rr * G + gl, nLm1 = (size_t)g.nL - 1;
+    const size_t cend = beta < nLm1 ? (beta + 2) * s : (size_t)cnt;
+    const size_t r0 = beta * s + 8 * gam;
+    const bool live = gl < G && r0 < (size_t)cnt;
+    const int len = live ? (int)(cend - r0) : 0;
+    const int h0 = (int)(r0 - hc0);
+    const double *pw = lw + gl * ly.LW + 16 * gam + sg;
+    const double *xsg = xs + sg * ly.nxs;
+    const int a0 = std::min(t0, len), a1 = t1 < 0 ? len : std::min(t1, len);
+    if ((size_t)(Rr + 1) * G > nLm1)  // the work-group holds the last chunk
+      band_seg8<DIR, PRE, true, true>(xsg, pw, hd, h0, sh, a0, a1, len, acc);
+    else
+      band_seg8<DIR, PRE, false, true>(xsg, pw, hd, h0, sh, a0, a1, len, acc);
+  };
+  // the sums of pair p's row group of this lane into staging st (= or +=)
+  auto stage_group = [&](int p, const double acc[8], double *st, bool add) {
+    // a fresh copy of the lane index (the empty asm hides that it equals
+    // the one the band used), so that the band's row indices are recomputed
+    // here instead of being held in registers through the band
+    int ln = lane;
+    asm volatile("" : "+v"(ln));
+    const int gl = ln & 15, sg = (ln >> 4) & 1;
+    const int cnt = (N - sg + 1) / 2;
+    const size_t r0 = ((size_t)Rr * G + gl) * s + 8 * (2 * p + (ln >> 5));
+    if (gl >= G || r0 >= (size_t)cnt) return;
+    const int rl = (int)(r0 - hc0);
+#pragma unroll
+    for (int q = 0; q < 8; q++)
+      if (r0 + q < (size_t)cnt) {
+        double &o = st[slot(2 * (rl + q) + sg)];
+        o = add ? o + acc[q] : acc[q];
+      }
+  };
+  if (threadIdx.x == 0) {
+    ctr[0] = 0;
+    ctr[1] = 0;
+  }
+  lds_sync();  // hardware barrier 1 of 2
+
+  if (tree) {
+    // the tree role's chain of dependent LDS and MFMA steps is the critical
+    // path: its waves issue ahead of the band role's FMA stream
+    __builtin_amdgcn_s_setprio(3);
+    RoleSync rs{ctr, nw, 0};
+    const double *m2l = wsall + blockIdx.y * wsstride + ws_layout(g).m2l;
+    const int nm = (g.ld + 2 * G - 1) * M2;
+    constexpr int QM = 6;
+    double vm[QM];
+    bool okm[QM];
+    const int nq = g.ld * M;
+    double qr[MT > 0 ? MT : 1];
+    constexpr int QD = MT > 0 ? (2 * MT * MT + 63) / 64 : 1;
+    double vd[QD];
+#pragma unroll
+    for (int q = 0; q < QM; q++) {
+      const int e = q * NT + tid;
+      const size_t a = m2l_elem(g, Rr, e < nm ? e : 0, M2, okm[q]);
+      okm[q] = okm[q] && e < nm;
+      vm[q] = m2l[okm[q] ? a : 0];
+    }
+    if (MT > 0 && nq > 0) {
+      const double *qrow = qrow_ptr(Qg, g, Rr, tid < nq ? tid : 0, M);
+#pragma unroll
+      for (int l = 0; l < (MT > 0 ? MT : 1); l++) qr[l] = qrow[l * M];
+    }
+#pragma unroll
+    for (int q = 0; q < QD; q++) {
+      const int e = q * NT + tid;
+      vd[q] = Dg[e < 2 * MM ? e : 0];
+    }
+#pragma unroll
+    for (int q = 0; q < QM; q++) {
+      const int e = q * NT + tid;
+      if (e < nm) mres[e] = okm[q] ? vm[q] : 0.0;
+    }
+    for (int e = QM * NT + tid; e < nm; e += NT) {
+      bool ok;
+      const size_t a = m2l_elem(g, Rr, e, M2, ok);
+      mres[e] = ok ? m2l[a] : 0.0;
+    }
+#pragma unroll
+    for (int q = 0; q < QD; q++) {
+      const int e = q * NT + tid;
+      if (MT > 0 && e < 2 * MM) dl[(e / M) * (M + 1) + e % M] = vd[q];
+    }
+    for (int e = (MT > 0 ? QD * NT : 0) + tid; e < 2 * MM; e += NT)
+      dl[(e / M) * (M + 1) + e % M] = Dg[e];
+    rs();
+    subtree_locals<MT, true>(g, Rr, tid, NT, mres, qp, cres, qr, nullptr, nullptr, Qg, Dg, dl,
+                             rs);
+    // far field: F_par^T (G x s) = C_par^T (G x M) T_par^T (M x s) on
+    // v_mfma_f64_16x16x4f64, two tiles (parity, 16 rows) per wave, written
+    // to obf (the chain and subtree operands under it are dead: the last
+    // role barrier of subtree_locals is behind every wave)
+    const int ntm = s >> 4, ksteps = (M + 3) >> 2;
+    constexpr int QK = MT > 0 ? (MT + 3) / 4 : 1;
+    const int n = lane & 15;
+#pragma unroll
+    for (int j = 0; j < 2; j++) {
+      const int tile = wave + j * nw, sgf = tile / ntm, rt = tile - sgf * ntm;
+      d4 D = {0, 0, 0, 0};
+      if (MT > 0) {
+        double av[QK], bv[QK];
+#pragma unroll
+        for (int kk = 0; kk < QK; kk++) {
+          const int k = kk * 4 + (lane >> 4);
+          bv[kk] = k < M ? Tg[((size_t)sgf * s + rt * 16 + n) * M + k] : 0.0;
+          av[kk] = (n < G && k < M) ? cres[(G - 1 + n) * M2 + sgf * M + k] : 0.0;
+        }
+#pragma unroll
+        for (int kk = 0; kk < QK; kk++) D = __builtin_amdgcn_mfma_f64_16x16x4f64(av[kk], bv[kk], D, 0, 0, 0);
+      } else {
+        for (int kk = 0; kk < ksteps; kk++) {
+          const int k = kk * 4 + (lane >> 4);
+          const double bv = k < M ? Tg[((size_t)sgf * s + rt * 16 + n) * M + k] : 0.0;
+          const double av = (n < G && k < M) ? cres[(G - 1 + n) * M2 + sgf * M + k] : 0.0;
+          D = __builtin_amdgcn_mfma_f64_16x16x4f64(av, bv, D, 0, 0, 0);
+        }
+      }
+#pragma unroll
+      for (int v = 0; v < 4; v++) {
+        const int leaf = (lane >> 4) + 4 * v, m = rt * 16 + n;
+        const int e = 2 * (leaf * s + m) + sgf;
+        if (leaf < G && e < nr) obf[slot(e)] = D[v];
+      }
+    }
+    rs();  // far field staged
+    // segment B of pair nw - 1 - wave (columns s - 8p - shift onward,
+    // k_down8's split moved by shift, a multiple of 8 in [-s, 8 nw]), once
+    // the band role has the windows in LDS; added into obf
+    const int p = nw - 1 - wave;
+    double acc[8];
+#pragma unroll
+    for (int q = 0; q < 8; q++) acc[q] = 0.0;
+    role_wait(ctr + 1, nw);
+    band(p, s - 8 * p - shift, -1, acc);
+    stage_group(p, acc, obf, true);
+    role_arrive(ctr + 1);  // done with the windows
+  } else {
+    RoleSync rs{ctr + 1, nw, 0};
+    // ---- the windows: x (both parities), the Lambda span, the head
+    const int nX = 2 * ((G + 1) * s + 8);
+    constexpr int QX = 9;  // ceil(nX / NT) for G = 16, s = 64
+    double vx[QX];
+    const int QL = 3 * s + 16, nLs = 2 * G * s + s + 16;
+    constexpr int QLS = 9;
+    double vl[QLS];
+    const int nhd = 2 * s + 16;
+#pragma unroll
+    for (int q = 0; q < QX; q++) vx[q] = src[std::min(row0 + q * NT + tid, (size_t)Nl - 1) * lm.es];
+#pragma unroll
+    for (int q = 0; q < QLS; q++) vl[q] = tab[std::min(2 * hc0 + q * NT + tid, (size_t)N - 1)];
+    double vh[3];
+#pragma unroll
+    for (int q = 0; q < 3; q++) vh[q] = head[q * NT + tid < nhd ? q * NT + tid : 0];
+#pragma unroll
+    for (int q = 0; q < QX; q++) {
+      const int e = q * NT + tid;
+      if (e < nX) {
+        const size_t i = row0 + e;
+        const double u = i < (size_t)Nl ? vx[q] : 0.0;
+        const int h = e >> 1;
+        xs[(e & 1) * ly.nxs + h + (h >> sh)] = (DIR == 1 && i > 0) ? u * (double)i : u;
+      }
+    }
+    for (int e = QX * NT + tid; e < nX; e += NT) {
+      const size_t i = row0 + e;
+      double u = i < (size_t)Nl ? src[i * lm.es] : 0.0;
+      if (DIR == 1 && i > 0) u *= (double)i;
+      const int h = e >> 1;
+      xs[(e & 1) * ly.nxs + h + (h >> sh)] = u;
+    }
+    auto put_lambda = [&](int e, double v) {
+      const int c = e >> (sh + 1), q = e - (c << (sh + 1));
+      if (2 * hc0 + e >= (size_t)N) v = 0.0;
+      if (c < G) lw[c * ly.LW + q] = v;
+      if (c > 0 && q + 2 * s < QL) lw[(c - 1) * ly.LW + q + 2 * s] = v;
+    };
+#pragma unroll
+    for (int q = 0; q < QLS; q++) {
+      const int e = q * NT + tid;
+      if (e < nLs) put_lambda(e, vl[q]);
+    }
+    for (int e = QLS * NT + tid; e < nLs; e += NT)
+      put_lambda(e, tab[std::min(2 * hc0 + e, (size_t)N - 1)]);
+#pragma unroll
+    for (int q = 0; q < 3; q++)
+      if (q * NT + tid < nhd) hd[q * NT + tid] = vh[q];
+    for (int e = 3 * NT + tid; e < nhd; e += NT) hd[e] = head[e];
+    rs();
+
+    // ---- segment A of pair wave - nw (columns [0, s - 8p)), the C2L
+    // diagonal with it
+    const int p = wave - nw;
+    double acc[8];
+#pragma unroll
+    for (int q = 0; q < 8; q++) acc[q] = 0.0;
+    if (DIR == 1) {  // C2L diagonal: sqrt(pi) an_i v_i (src/C/leg2cheb.c:416)
+      const int gl = lane & 15, sg = (lane >> 4) & 1, gam = 2 * p + (lane >> 5);
+      const size_t r0 = ((size_t)Rr * G + gl) * s + 8 * gam;
+      if (gl < G && r0 < (size_t)((N - sg + 1) / 2)) {
+        const int rl = (int)(r0 - hc0);
+        const double *xr = xs + sg * ly.nxs + rl + (rl >> sh);
+        const double *pw = lw + gl * ly.LW + 16 * gam + sg;
+#pragma unroll
+        for (int q = 0; q < 8; q++) acc[q] = kSqrtPi * xr[q] * pw[2 * q];
+      }
+    }
+    band(p, 0, s - 8 * p - shift, acc);
+    // windows dead once both roles' band waves are through: the staging
+    // takes their place
+    role_arrive(ctr + 1);
+    role_wait(ctr + 1, 3 * nw);
+    stage_group(p, acc, ob, false);
+  }
+  lds_sync();  // hardware barrier 2 of 2: the join
+  __builtin_amdgcn_s_setprio(0);
+
+  // ---- store: band plus far field, scaled
+  for (int e = threadIdx.x; e < nr; e += 2 * NT) {
+    const size_t i = row0 + e;
+    double v = ob[slot(e)] + obf[slot(e)];
     if (DIR == 0) {
       v *= k2Pi;
       if (i == 0) v *= 0.5;
@@ -1962,7 +2272,8 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) k
     dl[(e / M) * (M + 1) + e % M] = Dg[e];
   lds_sync();
 
-  subtree_locals<MT, true>(g, Rr, tid, NT, mres, qp, cres, qr, nullptr, nullptr, Qg, Dg, dl);
+  subtree_locals<MT, true>(g, Rr, tid, NT, mres, qp, cres, qr, nullptr, nullptr, Qg, Dg, dl,
+                           LdsSync{});
 
   // ---- far field: F_par^T (G x s) = C_par^T (G x M) T_par^T (M x s) on
   // v_mfma_f64_16x16x4f64, two tiles (parity, 16 rows) per wave
@@ -2524,7 +2835,19 @@ static void launch_down8(fmm_plan_s *p, const double *d_in, double *d_out, size_
   const double *tab = dir == L2C ? p->t.a : p->t.an;
   const double *htab = p->t.head[dir];
   prof_begin(p->prof, K_DOWN_BAND, st);
-  if (dir == L2C)
+  if (mode == K8_FULL && p->down_roles) {
+    const size_t shr = down8r_shared_bytes(g);
+    const int rsh = std::max(-g.s, std::min(p->roles_shift, 8 * (g.s >> 4))) & ~7;
+    if (dir == L2C)
+      launch_k(p->prof, k_down8r<0, 0, MT>, gd, 2 * nt, shr, st, d_in, d_out, lm, g, ws, wsstride,
+          p->t.D, p->t.Q, p->t.T, tab, htab, rsh);
+    else if (p->precompute)
+      launch_k(p->prof, k_down8r<1, 1, MT>, gd, 2 * nt, shr, st, d_in, d_out, lm, g, ws, wsstride,
+          p->t.D, p->t.Q, p->t.T, tab, htab, rsh);
+    else
+      launch_k(p->prof, k_down8r<1, 0, MT>, gd, 2 * nt, shr, st, d_in, d_out, lm, g, ws, wsstride,
+          p->t.D, p->t.Q, p->t.T, tab, htab, rsh);
+  } else if (dir == L2C)
     launch_k(p->prof, k_down8<0, 0, MT>, gd, nt, shm, st, d_in, d_out, lm, g, mode, ws, wsstride,
         p->t.D, p->t.Q, p->t.T, tab, htab);
   else if (p->precompute)
@@ -2574,11 +2897,12 @@ static void launch_far(fmm_plan_s *p, double *d_out, size_t dir, const LineMap &
 }
 
 // Whether this call takes the split downward pass (k_band + k_far): always
-// (down_split 1), never (0), or (2, the default) for calls of several lines
-// of deep trees (L >= 10, N >= 2^18 at s = 64), where its leaner kernels
-// overlap better: 8 lines of 2^20 in 293 us against 315 us fused, 32 lines
-// in 1017 against 1116 us; one line 58 against 56 us, 32 lines of 2^16 92
-// against 89 us.
+// (down_split 1), never (0, the default), or (2) for calls of several lines
+// of deep trees (L >= 10, N >= 2^18 at s = 64).  Against k_down8 the split
+// pass wins on such line calls (8 lines of 2^20 in 293 us against 315 us,
+// 32 lines in 1017 against 1116 us) and loses on single lines (58 against
+// 56 us at 2^20); k_down8r beats both (8 lines 286 us, 32 lines 1187
+// against 1230 us for the split pass in the same run, one line 48.6 us).
 static bool use_split(const fmm_plan_s *p, size_t nlines) {
   return p->geo.fast &&
          (p->down_split == 1 || (p->down_split == 2 && nlines > 1 && p->geo.L >= 10));

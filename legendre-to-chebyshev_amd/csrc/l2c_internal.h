@@ -93,6 +93,40 @@ __device__ __forceinline__ void lds_sync() {
   __builtin_amdgcn_s_barrier();
   __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup", "local");
 }
+struct LdsSync {
+  __device__ __forceinline__ void operator()() const { lds_sync(); }
+};
+
+// Synchronisation among the waves of one work-group that run different code
+// between two hardware barriers (wave roles), on LDS counters: a wave
+// arrives by adding one (after its LDS writes are complete) and waits until
+// a counter reaches a target.  The wait is bounded (2^22 polls, far beyond
+// any real wait), so that a wave can never spin forever.  LDS-only fences,
+// like lds_sync.
+__device__ __forceinline__ void role_arrive(int *ctr) {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup", "local");
+  if ((threadIdx.x & 63) == 0)
+    __hip_atomic_fetch_add(ctr, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+}
+__device__ __forceinline__ void role_wait(const int *ctr, int target) {
+  for (int k = 0; k < (1 << 22); k++) {
+    if (__hip_atomic_load(ctr, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP) >= target) break;
+    __builtin_amdgcn_s_sleep(1);
+  }
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup", "local");
+}
+// A barrier among the nw waves of one role: the k-th call returns once every
+// wave of the role has made its k-th call.
+struct RoleSync {
+  int *ctr;
+  int nw;
+  int target;
+  __device__ __forceinline__ void operator()() {
+    target += nw;
+    role_arrive(ctr);
+    role_wait(ctr, target);
+  }
+};
 
 // Line l of a batch starts at (l / n0) * ls1 + (l % n0) * ls0; its element i
 // is at that offset + i * es.  A single transform is {1, 0, 0, stride}.
@@ -238,9 +272,17 @@ struct fmm_plan_s {
   // M2L of levels >= lu on FP64 MFMA for calls of at least this many lines
   // (0: never); L2C_M2L_MMA sets it at plan creation
   int m2l_mma_lines = 0;
-  // downward pass of s = 16/32/64 plans: 0 the fused k_down8, 1 k_band +
-  // k_far, 2 (default) k_band + k_far for calls of several lines with L >= 10
-  int down_split = 2;
+  // downward pass of s = 16/32/64 plans: 0 (default) one fused kernel, 1
+  // k_band + k_far, 2 k_band + k_far for calls of several lines with L >= 10;
+  // L2C_DOWN_SPLIT
+  int down_split = 0;
+  // the fused downward pass with wave roles (k_down8r: tree and far field on
+  // half the waves, the band on the other half; default) or without (0:
+  // k_down8); L2C_DOWN_ROLES
+  int down_roles = 1;
+  // band columns moved from k_down8r's band role to its tree role (negative:
+  // the other way); L2C_ROLES_SHIFT
+  int roles_shift = -32;
   l2c::Geo geo{};
   l2c::DevTables t;
   direct_plan_s *dplan = nullptr;  // L < 1: direct path only

@@ -588,6 +588,8 @@ fmm_plan_s *create_fmm_impl(size_t N, size_t maxs, size_t M, size_t direction, s
   p->precompute = precompute;
   if (const char *e = getenv("L2C_DOWN_SPLIT")) p->down_split = std::max(0, std::min(2, atoi(e)));
   if (const char *e = getenv("L2C_M2L_MMA")) p->m2l_mma_lines = std::max(0, atoi(e));
+  if (const char *e = getenv("L2C_DOWN_ROLES")) p->down_roles = atoi(e) != 0;
+  if (const char *e = getenv("L2C_ROLES_SHIFT")) p->roles_shift = atoi(e);
   (void)hipGetDevice(&p->device);
   if (hipStreamCreateWithFlags(&p->stream, hipStreamNonBlocking) != hipSuccess) {
     set_error("hipStreamCreate failed");

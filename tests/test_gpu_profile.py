@@ -61,12 +61,15 @@ def test_profile_period(amd):
     assert {k: c for k, (_, c) in prof.items()} == {"k_up": 3, "k_m2l": 3, "k_down": 3}
 
 
-def test_profile_split_lines(amd):
-    """Lines of a deep tree take the split downward pass: k_band and k_far
-    each count one launch per call, and no k_down."""
+def test_profile_split_lines(amd, monkeypatch):
+    """Lines of a deep tree through the split downward pass (L2C_DOWN_SPLIT=2
+    at plan creation): k_band and k_far each count one launch per call, and
+    no k_down."""
     N, nlines, calls = 1 << 18, 4, 5
     announce(f"profile split lines N={N} nlines={nlines}")
+    monkeypatch.setenv("L2C_DOWN_SPLIT", "2")
     p = amd.FMMPlan(N, 64, 18, 0)
+    monkeypatch.delenv("L2C_DOWN_SPLIT")
     ref, _ = _run(amd, p, N, 1, nlines=nlines)
     p.profile(True)
     out, wall_ms = _run(amd, p, N, calls, nlines=nlines)
