@@ -113,9 +113,11 @@ def _counter_means(csv_dir, counter, copy_kib):
                     name = name[5:]
                 if name.startswith("l2c::"):
                     name = name[5:]
-                # profile slots: k_down covers both downward kernels (k_down8, k_down)
-                for tag, slot in (("k_up", "k_up"), ("k_m2l", "k_m2l"), ("k_down8", "k_down"),
-                                  ("k_down", "k_down"), ("k_direct", "k_direct")):
+                # profile slots: k_down covers the downward kernels (k_down8r,
+                # k_down8, k_down)
+                for tag, slot in (("k_up", "k_up"), ("k_m2l", "k_m2l"), ("k_down8r", "k_down"),
+                                  ("k_down8", "k_down"), ("k_down", "k_down"),
+                                  ("k_direct", "k_direct")):
                     if name.startswith(tag + "<") or name.startswith(tag + "(") or name == tag:
                         name = slot
                         break

@@ -1774,7 +1774,20 @@ __global__ void __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(4))) k
   const size_t off = line_off(lm, blockIdx.y);
   const double *src = in + off;
   double *dst = out + off;
-  const Down8rLds ly = down8r_lds(g);
+  // the layout is work-group uniform; its divisions run on the VALU, so its
+  // fields are moved to scalar registers rather than held (and spilled) in
+  // vector registers through the band loops
+  Down8rLds ly = down8r_lds(g);
+  ly.nxs = __builtin_amdgcn_readfirstlane(ly.nxs);
+  ly.lw = __builtin_amdgcn_readfirstlane(ly.lw);
+  ly.LW = __builtin_amdgcn_readfirstlane(ly.LW);
+  ly.hd = __builtin_amdgcn_readfirstlane(ly.hd);
+  ly.cres = __builtin_amdgcn_readfirstlane(ly.cres);
+  ly.dl = __builtin_amdgcn_readfirstlane(ly.dl);
+  ly.mres = __builtin_amdgcn_readfirstlane(ly.mres);
+  ly.qp = __builtin_amdgcn_readfirstlane(ly.qp);
+  ly.obf = __builtin_amdgcn_readfirstlane(ly.obf);
+  ly.ctr = __builtin_amdgcn_readfirstlane(ly.ctr);
   double *xs = sm + ly.xs, *lw = sm + ly.lw, *hd = sm + ly.hd;
   double *cres = sm + ly.cres, *dl = sm + ly.dl, *mres = sm + ly.mres, *qp = sm + ly.qp;
   double *ob = xs, *obf = sm + ly.obf;
@@ -1979,24 +1992,27 @@ __global__ void __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(4))) k
     for (int e = 3 * NT + tid; e < nhd; e += NT) hd[e] = head[e];
     rs();
 
-    // ---- segment A of pair wave - nw (columns [0, s - 8p)), the C2L
-    // diagonal with it
+    // ---- segment A of pair wave - nw (columns [0, s - 8p - shift)), then the
+    // C2L diagonal (added after the band: zero-started sums keep a single
+    // set of accumulators live through the band's two code paths)
     const int p = wave - nw;
     double acc[8];
 #pragma unroll
     for (int q = 0; q < 8; q++) acc[q] = 0.0;
+    band(p, 0, s - 8 * p - shift, acc);
     if (DIR == 1) {  // C2L diagonal: sqrt(pi) an_i v_i (src/C/leg2cheb.c:416)
-      const int gl = lane & 15, sg = (lane >> 4) & 1, gam = 2 * p + (lane >> 5);
+      int ln = lane;  // a fresh copy (see stage_group)
+      asm volatile("" : "+v"(ln));
+      const int gl = ln & 15, sg = (ln >> 4) & 1, gam = 2 * p + (ln >> 5);
       const size_t r0 = ((size_t)Rr * G + gl) * s + 8 * gam;
       if (gl < G && r0 < (size_t)((N - sg + 1) / 2)) {
         const int rl = (int)(r0 - hc0);
         const double *xr = xs + sg * ly.nxs + rl + (rl >> sh);
         const double *pw = lw + gl * ly.LW + 16 * gam + sg;
 #pragma unroll
-        for (int q = 0; q < 8; q++) acc[q] = kSqrtPi * xr[q] * pw[2 * q];
+        for (int q = 0; q < 8; q++) acc[q] += kSqrtPi * xr[q] * pw[2 * q];
       }
     }
-    band(p, 0, s - 8 * p - shift, acc);
     // windows dead once both roles' band waves are through: the staging
     // takes their place
     role_arrive(ctr + 1);
