@@ -2821,12 +2821,25 @@ static int launch_direct_lines(const direct_plan_s *dp, const double *d_in, doub
 // ---------------------------------------------------------------------------
 // host orchestration
 
+// Whether a call of nlines lines runs its M2L below lu on MFMA (precision
+// form 2, or FP64 MFMA for line calls from m2l_mma_lines lines on).
+static bool m2l_on_mfma(const fmm_plan_s *p, size_t nlines) {
+  return p->geo.M == 18 && (p->precision == 2 ||
+                            (p->m2l_mma_lines > 0 && nlines >= (size_t)p->m2l_mma_lines));
+}
+// The work-group geometry of a call: single-line calls take geo1, line
+// calls geo (the MFMA M2L forms, whose form-2 tables are cut at geo's lu,
+// always take geo).
+static const Geo &call_geo(const fmm_plan_s *p, size_t nlines) {
+  return nlines == 1 && !m2l_on_mfma(p, nlines) ? p->geo1 : p->geo;
+}
+
 // Workspace for a batch of lines: grown on demand (at least doubling), owned
 // by the plan.  Earlier calls may still be running on a caller's stream, so
 // the store being replaced is retired, not freed: the plan releases it when
 // it is itself freed, and nothing waits here on work of other streams.
-static int ensure_ws(fmm_plan_s *p, size_t nlines, double **w) {
-  const size_t need = std::max<size_t>(nlines, 1) * ws_doubles(p->geo);
+static int ensure_ws(fmm_plan_s *p, size_t nlines, const Geo &g, double **w) {
+  const size_t need = std::max<size_t>(nlines, 1) * ws_doubles(g);
   if (p->wl_cap < need) {
     const size_t cap = std::max(need, 2 * p->wl_cap);
     double *nw = nullptr;
@@ -2841,9 +2854,8 @@ static int ensure_ws(fmm_plan_s *p, size_t nlines, double **w) {
 
 // k_down8 launch in the given mode on stream st
 template <int MT>
-static void launch_down8(fmm_plan_s *p, const double *d_in, double *d_out, size_t dir,
+static void launch_down8(fmm_plan_s *p, const Geo &g, const double *d_in, double *d_out, size_t dir,
                          const LineMap &lm, size_t nlines, hipStream_t st, int mode, double *ws) {
-  const Geo &g = p->geo;
   const size_t wsstride = ws_doubles(g);
   dim3 gd((unsigned)((size_t)4 << g.ld), (unsigned)nlines);
   const size_t shm = down_shared_bytes(g);
@@ -2878,9 +2890,8 @@ static void launch_down8(fmm_plan_s *p, const double *d_in, double *d_out, size_
 // The split downward pass on st: k_band (before the tree kernels, which it
 // does not need), then k_up and k_m2l, then k_far.
 template <int MT>
-static void launch_band(fmm_plan_s *p, const double *d_in, double *d_out, size_t dir,
+static void launch_band(fmm_plan_s *p, const Geo &g, const double *d_in, double *d_out, size_t dir,
                         const LineMap &lm, size_t nlines, hipStream_t st, int scale) {
-  const Geo &g = p->geo;
   dim3 gd((unsigned)((size_t)4 << g.ld), (unsigned)nlines);
   const size_t shm = band_shared_bytes(g);
   const int nt = 128 * (g.s >> 4);
@@ -2897,9 +2908,8 @@ static void launch_band(fmm_plan_s *p, const double *d_in, double *d_out, size_t
 }
 
 template <int MT>
-static void launch_far(fmm_plan_s *p, double *d_out, size_t dir, const LineMap &lm,
+static void launch_far(fmm_plan_s *p, const Geo &g, double *d_out, size_t dir, const LineMap &lm,
                        size_t nlines, hipStream_t st, double *ws) {
-  const Geo &g = p->geo;
   dim3 gd((unsigned)((size_t)4 << g.ld), (unsigned)nlines);
   const int nt = 64 * (g.s >> 4);
   prof_begin(p->prof, K_FAR, st);
@@ -2926,10 +2936,9 @@ static bool use_split(const fmm_plan_s *p, size_t nlines) {
 
 // One execute (or a batch of lines): upward pass, M2L, downward pass, all on st.
 template <int MT>
-static int launch_fmm_m(fmm_plan_s *p, const double *d_in, double *d_out, size_t dir,
-                        const LineMap &lm, size_t nlines, hipStream_t st, int band_only,
-                        double *ws) {
-  const Geo &g = p->geo;
+static int launch_fmm_m(fmm_plan_s *p, const Geo &g, const double *d_in, double *d_out,
+                        size_t dir, const LineMap &lm, size_t nlines, hipStream_t st,
+                        int band_only, double *ws) {
   const size_t wsstride = ws_doubles(g);
   if (lm.n && !g.fast) {  // per-line lengths: k_up and k_down8 only
     set_error("lines of uneven length need a plan with leaf size s = 16, 32 or 64");
@@ -2937,7 +2946,7 @@ static int launch_fmm_m(fmm_plan_s *p, const double *d_in, double *d_out, size_t
   }
   const bool split = use_split(p, nlines);
   if (split) {
-    launch_band<MT>(p, d_in, d_out, dir, lm, nlines, st, band_only);
+    launch_band<MT>(p, g, d_in, d_out, dir, lm, nlines, st, band_only);
     if (band_only) return 0;
   }
   if (!band_only) {
@@ -2958,8 +2967,7 @@ static int launch_fmm_m(fmm_plan_s *p, const double *d_in, double *d_out, size_t
     const DevTables::Split &sp = p->t.sp[dir];
     const SplitView sv{p->precision == 1 ? sp.rec : nullptr, sp.pos, sp.n64, sp.n32, sp.n16,
                        sp.rec_bytes};
-    const bool mma = MT == 18 && (p->precision == 2 ||
-                                  (p->m2l_mma_lines > 0 && nlines >= (size_t)p->m2l_mma_lines));
+    const bool mma = MT == 18 && m2l_on_mfma(p, nlines);
     if (!mma) {
       launch_k(p->prof, k_m2l<MT>, gm, kM2lNT, m2l_shared_bytes(g), st, g, p->t.A[dir], ws,
           wsstride, bpw, nlines, lpb, sv);
@@ -2988,11 +2996,11 @@ static int launch_fmm_m(fmm_plan_s *p, const double *d_in, double *d_out, size_t
     prof_end(p->prof, st);
   }
   if (split) {
-    launch_far<MT>(p, d_out, dir, lm, nlines, st, ws);
+    launch_far<MT>(p, g, d_out, dir, lm, nlines, st, ws);
     return 0;
   }
   if (g.fast) {
-    launch_down8<MT>(p, d_in, d_out, dir, lm, nlines, st, band_only ? K8_BAND : K8_FULL, ws);
+    launch_down8<MT>(p, g, d_in, d_out, dir, lm, nlines, st, band_only ? K8_BAND : K8_FULL, ws);
     return 0;
   }
   dim3 gd((unsigned)((size_t)4 << g.ld), (unsigned)nlines);
@@ -3016,9 +3024,10 @@ static int launch_fmm_m(fmm_plan_s *p, const double *d_in, double *d_out, size_t
 static int launch_fmm(fmm_plan_s *p, const double *d_in, double *d_out, size_t dir,
                       const LineMap &lm, size_t nlines, hipStream_t st, int band_only) {
   double *ws = nullptr;
-  if (ensure_ws(p, nlines, &ws)) return -1;
-  const int rc = p->geo.M == 18 ? launch_fmm_m<18>(p, d_in, d_out, dir, lm, nlines, st, band_only, ws)
-                                : launch_fmm_m<0>(p, d_in, d_out, dir, lm, nlines, st, band_only, ws);
+  const Geo &g = call_geo(p, nlines);
+  if (ensure_ws(p, nlines, g, &ws)) return -1;
+  const int rc = g.M == 18 ? launch_fmm_m<18>(p, g, d_in, d_out, dir, lm, nlines, st, band_only, ws)
+                           : launch_fmm_m<0>(p, g, d_in, d_out, dir, lm, nlines, st, band_only, ws);
   if (rc) return rc;
   L2C_HIP(hipGetLastError(), -1);
   return 0;
@@ -3057,7 +3066,7 @@ int run_packed(fmm_plan_s *p, const direct_plan_s *dp, const double *d_in, doubl
     if (dp) {
       if (launch_direct_lines(dp, d_in, d_out, dir, lm, nl, st)) return -1;
     } else {
-      if (p->wl_cap < nl * ws_doubles(p->geo)) {
+      if (p->wl_cap < nl * ws_doubles(call_geo(p, nl))) {
         set_error("run_packed: workspace not reserved for this many lines");
         return -1;
       }
@@ -3089,7 +3098,8 @@ int build_slices(fmm_plan_s *p, int dir) {
 // graph capture included, never allocate).
 int reserve_lines(fmm_plan_s *p, size_t nlines) {
   double *w = nullptr;
-  return ensure_ws(p, std::min<size_t>(nlines, 65535), &w);
+  const Geo &g = ws_doubles(p->geo1) > ws_doubles(p->geo) ? p->geo1 : p->geo;
+  return ensure_ws(p, std::min<size_t>(nlines, 65535), g, &w);
 }
 
 int run_lines(fmm_plan_s *p, const double *d_in, double *d_out, size_t dir, size_t nlines,
@@ -3159,7 +3169,7 @@ static int run_lines_scratch(fmm_plan_s *p, double *scr, const double *d_in, dou
                              size_t es, hipStream_t st) {
   LineMap lm{n0 ? n0 : 1, ls0, ls1, es};
   if (p->L == 0) return launch_direct_lines(p->dplan, d_in, d_out, dir, lm, nlines, st);
-  const size_t wsd = ws_doubles(p->geo);
+  const size_t wsd = std::max(ws_doubles(p->geo), ws_doubles(p->geo1));
   const size_t per = std::min<size_t>(65535, kScratchDoubles / wsd);
   if (per == 0) {
     set_error("l2c_execute_lines_device_noalloc: one line of this plan needs " +
@@ -3170,8 +3180,9 @@ static int run_lines_scratch(fmm_plan_s *p, double *scr, const double *d_in, dou
   for (size_t l0 = 0; l0 < nlines; l0 += per) {
     lm.l0 = l0;
     const size_t nl = std::min(per, nlines - l0);
-    const int rc = p->geo.M == 18 ? launch_fmm_m<18>(p, d_in, d_out, dir, lm, nl, st, 0, scr)
-                                  : launch_fmm_m<0>(p, d_in, d_out, dir, lm, nl, st, 0, scr);
+    const Geo &g = call_geo(p, nl);
+    const int rc = g.M == 18 ? launch_fmm_m<18>(p, g, d_in, d_out, dir, lm, nl, st, 0, scr)
+                             : launch_fmm_m<0>(p, g, d_in, d_out, dir, lm, nl, st, 0, scr);
     if (rc) return rc;
     L2C_HIP(hipGetLastError(), -1);
   }

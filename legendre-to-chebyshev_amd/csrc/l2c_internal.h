@@ -283,7 +283,12 @@ struct fmm_plan_s {
   // band columns moved from k_down8r's band role to its tree role (negative:
   // the other way); L2C_ROLES_SHIFT
   int roles_shift = -32;
+  // work-group geometry of line calls (geo) and of single-line calls (geo1):
+  // the same tree, cut into subtrees of different size; the composite
+  // operators P and Q hold levels 1..lu_tab and 1..ld_tab, enough for both
   l2c::Geo geo{};
+  l2c::Geo geo1{};
+  int lu_tab = 0, ld_tab = 0;
   l2c::DevTables t;
   direct_plan_s *dplan = nullptr;  // L < 1: direct path only
   double *d_w = nullptr;           // multipole store: slots x 2 x M

@@ -539,13 +539,25 @@ static void destroy_fmm(fmm_plan_s *p) {
 
 // Work-group shapes: leaf chunks per upward / downward work-group, shrunk
 // until the work-group's LDS fits the budget.
-static bool choose_cuts(Geo &g) {
+// single: the cuts of single-line calls, else those of line calls.  Line
+// calls (their grids hold every line's work-groups) take the largest
+// subtrees; single transforms take cuts that grow with the depth, gu = gd =
+// (L - 4) / 2 within [1, 4] (gd >= 2 for k_down8r), measured best at
+// L = 4 ... 12 (s = 64: 2^16 23.1 us at (2, 2) against 26.8 us at (4, 4);
+// 2^18 26.6 against 28.6 us at (3, 3); 2^20 (4, 4) best), while line calls
+// lose with them (64 lines of 2^16: 174 K against 432 K lines/s).
+static bool choose_cuts(Geo &g, bool single) {
   const int Lm1 = g.L - 1;
   // k_down8 (s = 16, 32, 64; L2C_DOWN8=0 selects k_down): 16 leaf chunks and
   // up to 80 KiB of LDS (two work-groups per CU); k_down: 8 and 64 KiB
   const char *e8 = getenv("L2C_DOWN8");
   g.fast = down8_eligible(g) && !(e8 && atoi(e8) == 0) ? 1 : 0;
+  const int dcut = (g.L - 4) / 2;
   int gu = 4, gd = g.fast ? 4 : 3;
+  if (single) {
+    gu = std::max(1, std::min(4, dcut));
+    if (g.fast) gd = std::max(2, std::min(4, dcut));
+  }
   if (const char *e = getenv("L2C_GU")) gu = atoi(e);
   if (const char *e = getenv("L2C_GD")) gd = atoi(e);
   // k_up holds its composite operators in LDS (lu of them): up to 80 KiB
@@ -634,11 +646,15 @@ fmm_plan_s *create_fmm_impl(size_t N, size_t maxs, size_t M, size_t direction, s
   g.M = (int)M;
   g.nL = (int)(N / (2 * s));
   g.nodal = lagrange ? 1 : 0;
-  if (!choose_cuts(g)) {
+  p->geo1 = g;
+  if (!choose_cuts(g, false) || !choose_cuts(p->geo1, true)) {
     set_error("create_fmm: leaf size s and rank M need more than 64 KiB of LDS per work-group");
     destroy_fmm(p);
     return nullptr;
   }
+  // the composite operators serve both geometries
+  p->lu_tab = std::max(g.lu, p->geo1.lu);
+  p->ld_tab = std::max(g.ld, p->geo1.ld);
   if (v > 1) {
     printf("N %zu\nNum levels %d\nNum submatrices %zu\nNum blocks %zu\n", N, L, p->Nc, p->Nb);
     printf("Given max s %zu \nComputed s %zu \nComputed N %zu\nLagrange %zu\n", maxs, s, Nn,
@@ -741,7 +757,7 @@ fmm_plan_s *create_fmm_impl(size_t N, size_t maxs, size_t M, size_t direction, s
         D[q * MM + i * M + j] = U[q * MM + j * M + i];  // D(p) = U(p)^T
   // composite operators P(k, d), k = 1..lu, d < 2^k:
   // P(1,q) = U(q), P(k,d) = U(d >> (k-1)) P(k-1, d mod 2^(k-1))
-  const int lu = g.lu;
+  const int lu = p->lu_tab;
   const size_t nP = lu > 0 ? (((size_t)2 << lu) - 2) : 0;
   std::vector<double> P(std::max<size_t>(nP, 1) * MM);
   for (int k = 1; k <= lu; k++) {
@@ -760,7 +776,7 @@ fmm_plan_s *create_fmm_impl(size_t N, size_t maxs, size_t M, size_t direction, s
   // composite downward operators Q(k, d), k = 1..ld, d < 2^k: the L2L chain
   // from a level-l chunk down k levels to its descendant whose low k index
   // bits are d.  Q(1,q) = D(q), Q(k,d) = D(d & 1) Q(k-1, d >> 1).
-  const int ldq = g.ld;
+  const int ldq = p->ld_tab;
   const size_t nQ = ldq > 0 ? (((size_t)2 << ldq) - 2) : 0;
   std::vector<double> Q(std::max<size_t>(nQ, 1) * MM);
   for (int k = 1; k <= ldq; k++) {
@@ -820,7 +836,7 @@ fmm_plan_s *create_fmm_impl(size_t N, size_t maxs, size_t M, size_t direction, s
   // multipole store and staging
   const size_t slots = slot_base(L);
   (void)slots;
-  p->wl_cap = ws_doubles(g);
+  p->wl_cap = std::max(ws_doubles(g), ws_doubles(p->geo1));
   // (host staging buffers are allocated by the first call with host arrays)
   if (dmalloc(&p->d_wl, p->wl_cap, acct)) {
     destroy_fmm(p);
@@ -879,8 +895,8 @@ int l2c_get_plan_info(const fmm_plan *p, l2c_plan_info *info) {
   info->Nb = p->Nb;
   info->hbm_bytes = p->hbm_bytes;
   info->a_bytes = p->Nc * p->M * p->M * sizeof(double);
-  info->subtree_up = p->L ? 1 << p->geo.gu : 0;
-  info->subtree_down = p->L ? 1 << p->geo.gd : 0;
+  info->subtree_up = p->L ? 1 << p->geo1.gu : 0;
+  info->subtree_down = p->L ? 1 << p->geo1.gd : 0;
   return 0;
 }
 
@@ -930,12 +946,12 @@ size_t l2c_get_plan_table(const fmm_plan *p, int table, size_t direction, double
       n = 2 * MM;
       break;
     case L2C_TAB_P:
-      src = fmm && p->geo.lu > 0 ? p->t.P : nullptr;
-      n = fmm && p->geo.lu > 0 ? (((size_t)2 << p->geo.lu) - 2) * MM : 0;
+      src = fmm && p->lu_tab > 0 ? p->t.P : nullptr;
+      n = fmm && p->lu_tab > 0 ? (((size_t)2 << p->lu_tab) - 2) * MM : 0;
       break;
     case L2C_TAB_Q:
-      src = fmm && p->geo.ld > 0 ? p->t.Q : nullptr;
-      n = fmm && p->geo.ld > 0 ? (((size_t)2 << p->geo.ld) - 2) * MM : 0;
+      src = fmm && p->ld_tab > 0 ? p->t.Q : nullptr;
+      n = fmm && p->ld_tab > 0 ? (((size_t)2 << p->ld_tab) - 2) * MM : 0;
       break;
     default:
       set_error("l2c_get_plan_table: unknown table");
