@@ -40,11 +40,13 @@
 // Latency, not bandwidth, bounds a work-group here: every global load a
 // work-group needs (its data and its twiddles) is issued at the start, the
 // in-FFT twiddles come from a quarter-period table in LDS, and each thread
-// owns 8 elements (threads per work-group = elements / 8).  All arithmetic is
+// owns 8 elements (radix-8 stages) or, in the passes of small jobs, 4
+// (radix-4 stages, twice the waves per work-group).  All arithmetic is
 // FP64; every twiddle is a correctly rounded table entry (built on the host
 // in extended precision) or the product of two of them.
 #include <algorithm>
 #include <cmath>
+#include <cstdlib>
 #include <cstring>
 
 #include "l2c_internal.h"
@@ -193,14 +195,15 @@ __device__ __forceinline__ void fft_stage(double2 *buf, const double2 *ts) {
 }
 
 // FFT (S = -1) or unnormalised inverse FFT (S = +1) of NC LDS columns of
-// length 2^LOGL: a radix-2 or radix-4 stage first when LOGL is not a
-// multiple of 3, radix 8 after.  The caller synchronises before the call.
-template <int LOGL, int NC, int T, int S, int LOGNS = 0>
+// length 2^LOGL in stages of radix 2^LOGRM (8 or 4), a smaller stage first
+// when LOGL is not a multiple of LOGRM.  The caller synchronises before the
+// call.
+template <int LOGL, int NC, int T, int S, int LOGRM = 3, int LOGNS = 0>
 __device__ __forceinline__ void fft_cols(double2 *buf, const double2 *ts) {
   if constexpr (LOGNS < LOGL) {
-    constexpr int LOGR = (LOGNS == 0 && LOGL % 3) ? LOGL % 3 : 3;
+    constexpr int LOGR = (LOGNS == 0 && LOGL % LOGRM) ? LOGL % LOGRM : LOGRM;
     fft_stage<LOGL, NC, T, 1 << LOGR, LOGNS, S>(buf, ts);
-    fft_cols<LOGL, NC, T, S, LOGNS + LOGR>(buf, ts);
+    fft_cols<LOGL, NC, T, S, LOGRM, LOGNS + LOGR>(buf, ts);
   }
 }
 
@@ -350,10 +353,10 @@ __global__ void __launch_bounds__(kElems / 8)
 }
 
 // Pass 1: FFT length L = M2 over n2; NC columns n1 (reflect pairing).
-template <int LOGL, int NC>
-__global__ void __launch_bounds__(NC * (1 << LOGL) / 8)
+template <int LOGL, int NC, int PT>
+__global__ void __launch_bounds__(NC * (1 << LOGL) / PT)
     k_dct2_p1(const double *__restrict__ x, double2 *__restrict__ W, int logM1, Tabs tb) {
-  constexpr int L = 1 << LOGL, LD = L + 1, C = NC / 2, T = NC * L / 8;
+  constexpr int L = 1 << LOGL, LD = L + 1, C = NC / 2, T = NC * L / PT;
   extern __shared__ double2 z[];
   double2 *ts = z + NC * LD;
   const int M1 = 1 << logM1;
@@ -362,16 +365,16 @@ __global__ void __launch_bounds__(NC * (1 << LOGL) / 8)
   W += size_t(blockIdx.y) * M;
   const int a = chunk_index() * C;
   // store items: i = id / L, k2 = id % L; twiddle e^{-2 pi i n1 k2 / M}
-  double2 th[8], tl[8], p0[4], p1[4];
+  double2 th[PT], tl[PT], p0[PT / 2], p1[PT / 2];
 #pragma unroll
-  for (int it = 0; it < 8; ++it) {
+  for (int it = 0; it < PT; ++it) {
     const int id = threadIdx.x + it * T;
     const size_t n1 = size_t(col_reflect<C>(id / L, a, M1));
     pass_root_load(tb, (n1 * size_t(id % L)) & (M - 1), th[it], tl[it]);
   }
   // load items: i = id % NC, n2 = id / NC < L/2
 #pragma unroll
-  for (int it = 0; it < 4; ++it) {
+  for (int it = 0; it < PT / 2; ++it) {
     const int id = threadIdx.x + it * T, i = id % NC, n2 = id / NC;
     const size_t m = size_t(col_reflect<C>(i, a, M1)) + (size_t(n2) << logM1);
     p0[it] = *reinterpret_cast<const double2 *>(x + 4 * m);
@@ -379,15 +382,15 @@ __global__ void __launch_bounds__(NC * (1 << LOGL) / 8)
   }
   load_quarter<LOGL, T>(ts, tb.tw);
 #pragma unroll
-  for (int it = 0; it < 4; ++it) {
+  for (int it = 0; it < PT / 2; ++it) {
     const int id = threadIdx.x + it * T, i = id % NC, n2 = id / NC;
     z[i * LD + n2] = make_double2(p0[it].x, p1[it].x);
     z[partner<C>(i) * LD + (L - 1 - n2)] = make_double2(p1[it].y, p0[it].y);
   }
   __syncthreads();
-  fft_cols<LOGL, NC, T, -1>(z, ts);
+  fft_cols<LOGL, NC, T, -1, PT == 8 ? 3 : 2>(z, ts);
 #pragma unroll
-  for (int it = 0; it < 8; ++it) {
+  for (int it = 0; it < PT; ++it) {
     const int id = threadIdx.x + it * T, i = id / L, k2 = id % L;
     const size_t n1 = size_t(col_reflect<C>(i, a, M1));
     W[(n1 << LOGL) + k2] = cmul(z[i * LD + k2], cmul(th[it], tl[it]));
@@ -395,10 +398,10 @@ __global__ void __launch_bounds__(NC * (1 << LOGL) / 8)
 }
 
 // Pass 2: FFT length L = M1 over n1; NC columns k2 (negate pairing).
-template <int LOGL, int NC>
-__global__ void __launch_bounds__(NC * (1 << LOGL) / 8)
+template <int LOGL, int NC, int PT>
+__global__ void __launch_bounds__(NC * (1 << LOGL) / PT)
     k_dct2_p2(const double2 *__restrict__ W, double *__restrict__ y, int logM2, Tabs tb) {
-  constexpr int L = 1 << LOGL, LD = L + 1, C = NC / 2, T = NC * L / 8;
+  constexpr int L = 1 << LOGL, LD = L + 1, C = NC / 2, T = NC * L / PT;
   extern __shared__ double2 z[];
   double2 *ts = z + NC * LD;
   const int M2 = 1 << logM2;
@@ -410,28 +413,28 @@ __global__ void __launch_bounds__(NC * (1 << LOGL) / 8)
   const int i = threadIdx.x % NC;
   const int k2 = col_negate<C>(i, a, M2);
   const double2 c1 = tb.g1[k2], c4 = tb.g4[k2];
-  double2 r1[8], r4[8], d[8];
+  double2 r1[PT], r4[PT], d[PT];
 #pragma unroll
-  for (int it = 0; it < 8; ++it) {
+  for (int it = 0; it < PT; ++it) {
     const int k1 = (threadIdx.x + it * T) / NC;
     r1[it] = tb.h1[k1];
     r4[it] = tb.h4[k1];
   }
 #pragma unroll
-  for (int it = 0; it < 8; ++it) {
+  for (int it = 0; it < PT; ++it) {
     const int n1 = (threadIdx.x + it * T) / NC;
     d[it] = W[size_t(n1) * M2 + k2];
   }
   load_quarter<LOGL, T>(ts, tb.tw);
 #pragma unroll
-  for (int it = 0; it < 8; ++it) z[i * LD + (threadIdx.x + it * T) / NC] = d[it];
+  for (int it = 0; it < PT; ++it) z[i * LD + (threadIdx.x + it * T) / NC] = d[it];
   __syncthreads();
-  fft_cols<LOGL, NC, T, -1>(z, ts);
+  fft_cols<LOGL, NC, T, -1, PT == 8 ? 3 : 2>(z, ts);
   // the partner of column k2: itself for k2 = 0 (row (L - k1) mod L) and for
   // k2 = M2/2, else column partner(i); rows run backwards (L - 1 - k1)
   const int ip = (k2 == M2 / 2 || k2 == 0) ? i : partner<C>(i);
 #pragma unroll
-  for (int it = 0; it < 8; ++it) {
+  for (int it = 0; it < PT; ++it) {
     const int k1 = (threadIdx.x + it * T) / NC;
     const int row = k2 == 0 ? ((L - k1) & (L - 1)) : L - 1 - k1;
     dct2_post(y, size_t(k1) * M2 + k2, M, z[i * LD + k1], conj2(z[ip * LD + row]),
@@ -532,10 +535,10 @@ __global__ void __launch_bounds__(kElems / 8)
 
 // Pass 1: inverse FFT length L = M2 over n2; NC columns n1 (negate pairing).
 // Y is read as a 2 M2 x M1 row-major matrix.
-template <int LOGL, int NC>
-__global__ void __launch_bounds__(NC * (1 << LOGL) / 8)
+template <int LOGL, int NC, int PT>
+__global__ void __launch_bounds__(NC * (1 << LOGL) / PT)
     k_dct3_p1(const double *__restrict__ Y, double2 *__restrict__ W, int logM1, Tabs tb) {
-  constexpr int L = 1 << LOGL, LD = L + 1, C = NC / 2, T = NC * L / 8;
+  constexpr int L = 1 << LOGL, LD = L + 1, C = NC / 2, T = NC * L / PT;
   extern __shared__ double2 z[];
   double2 *ts = z + NC * LD;
   double *yr = reinterpret_cast<double *>(z);
@@ -548,28 +551,28 @@ __global__ void __launch_bounds__(NC * (1 << LOGL) / 8)
   const int i = threadIdx.x % NC;
   const int n1 = col_negate<C>(i, a, M1);
   const double2 c1 = tb.g1[n1], c4 = tb.g4[n1];
-  double2 r1[8], r4[8];
-  double q[16];
+  double2 r1[PT], r4[PT];
+  double q[2 * PT];
 #pragma unroll
-  for (int it = 0; it < 8; ++it) {
+  for (int it = 0; it < PT; ++it) {
     const int n2 = (threadIdx.x + it * T) / NC;
     r1[it] = tb.e1[n2];
     r4[it] = tb.e4[n2];
   }
 #pragma unroll
-  for (int it = 0; it < 16; ++it) {
+  for (int it = 0; it < 2 * PT; ++it) {
     const int r = (threadIdx.x + it * T) / NC;  // rows 0 .. 2L-1
     q[it] = Y[size_t(r) * M1 + n1];
   }
   load_quarter<LOGL, T>(ts, tb.tw);
 #pragma unroll
-  for (int it = 0; it < 16; ++it) yr[i * 2 * LD + (threadIdx.x + it * T) / NC] = q[it];
+  for (int it = 0; it < 2 * PT; ++it) yr[i * 2 * LD + (threadIdx.x + it * T) / NC] = q[it];
   __syncthreads();
   const double *c = yr + i * 2 * LD;
   const double *p = yr + (n1 == M1 / 2 || n1 == 0 ? i : partner<C>(i)) * 2 * LD;
-  double2 zp[8];
+  double2 zp[PT];
 #pragma unroll
-  for (int it = 0; it < 8; ++it) {
+  for (int it = 0; it < PT; ++it) {
     const int n2 = (threadIdx.x + it * T) / NC;
     double bb, dd;
     if (n1 == 0) {
@@ -583,19 +586,19 @@ __global__ void __launch_bounds__(NC * (1 << LOGL) / 8)
   }
   __syncthreads();
 #pragma unroll
-  for (int it = 0; it < 8; ++it) z[i * LD + (threadIdx.x + it * T) / NC] = zp[it];
+  for (int it = 0; it < PT; ++it) z[i * LD + (threadIdx.x + it * T) / NC] = zp[it];
   __syncthreads();
   // store items: i = id / L, k2 = id % L; twiddle e^{+2 pi i n1 k2 / M}
-  double2 th[8], tl[8];
+  double2 th[PT], tl[PT];
 #pragma unroll
-  for (int it = 0; it < 8; ++it) {
+  for (int it = 0; it < PT; ++it) {
     const int id = threadIdx.x + it * T;
     const size_t m1 = size_t(col_negate<C>(id / L, a, M1));
     pass_root_load(tb, (m1 * size_t(id % L)) & (M - 1), th[it], tl[it]);
   }
-  fft_cols<LOGL, NC, T, 1>(z, ts);
+  fft_cols<LOGL, NC, T, 1, PT == 8 ? 3 : 2>(z, ts);
 #pragma unroll
-  for (int it = 0; it < 8; ++it) {
+  for (int it = 0; it < PT; ++it) {
     const int id = threadIdx.x + it * T, j = id / L, k2 = id % L;
     const size_t m1 = size_t(col_negate<C>(j, a, M1));
     W[(m1 << LOGL) + k2] = cmul(z[j * LD + k2], conj2(cmul(th[it], tl[it])));
@@ -603,10 +606,10 @@ __global__ void __launch_bounds__(NC * (1 << LOGL) / 8)
 }
 
 // Pass 2: inverse FFT length L = M1 over n1; NC columns k2 (reflect pairing).
-template <int LOGL, int NC>
-__global__ void __launch_bounds__(NC * (1 << LOGL) / 8)
+template <int LOGL, int NC, int PT>
+__global__ void __launch_bounds__(NC * (1 << LOGL) / PT)
     k_dct3_p2(const double2 *__restrict__ W, double *__restrict__ x, int logM2, Tabs tb) {
-  constexpr int L = 1 << LOGL, LD = L + 1, C = NC / 2, T = NC * L / 8;
+  constexpr int L = 1 << LOGL, LD = L + 1, C = NC / 2, T = NC * L / PT;
   extern __shared__ double2 z[];
   double2 *ts = z + NC * LD;
   const int M2 = 1 << logM2;
@@ -616,17 +619,17 @@ __global__ void __launch_bounds__(NC * (1 << LOGL) / 8)
   const int a = chunk_index() * C;
   const int i = threadIdx.x % NC;
   const int k2 = col_reflect<C>(i, a, M2);
-  double2 d[8];
+  double2 d[PT];
 #pragma unroll
-  for (int it = 0; it < 8; ++it) d[it] = W[size_t((threadIdx.x + it * T) / NC) * M2 + k2];
+  for (int it = 0; it < PT; ++it) d[it] = W[size_t((threadIdx.x + it * T) / NC) * M2 + k2];
   load_quarter<LOGL, T>(ts, tb.tw);
 #pragma unroll
-  for (int it = 0; it < 8; ++it) z[i * LD + (threadIdx.x + it * T) / NC] = d[it];
+  for (int it = 0; it < PT; ++it) z[i * LD + (threadIdx.x + it * T) / NC] = d[it];
   __syncthreads();
-  fft_cols<LOGL, NC, T, 1>(z, ts);
+  fft_cols<LOGL, NC, T, 1, PT == 8 ? 3 : 2>(z, ts);
   const int ip = partner<C>(i);
 #pragma unroll
-  for (int it = 0; it < 4; ++it) {
+  for (int it = 0; it < PT / 2; ++it) {
     const int k1 = (threadIdx.x + it * T) / NC;  // < L/2
     const size_t m = size_t(k1) * M2 + k2;
     const double2 A = z[i * LD + k1], B = z[ip * LD + (L - 1 - k1)];
@@ -644,11 +647,11 @@ __global__ void __launch_bounds__(NC * (1 << LOGL) / 8)
 // which the last pass reads as L1 rows of L2 L3 columns.  A work-group owns
 // one n1 and NC consecutive k3, so every load and store is a run of NC
 // complex values.
-template <int LOGL, int NC, int S>
-__global__ void __launch_bounds__(NC * (1 << LOGL) / 8)
+template <int LOGL, int NC, int S, int PT>
+__global__ void __launch_bounds__(NC * (1 << LOGL) / PT)
     k_fft_mid(const double2 *__restrict__ W, double2 *__restrict__ W2, int logL1, int logL3,
               Tabs tb) {
-  constexpr int L = 1 << LOGL, LD = L + 1, T = NC * L / 8;
+  constexpr int L = 1 << LOGL, LD = L + 1, T = NC * L / PT;
   extern __shared__ double2 z[];
   double2 *ts = z + NC * LD;
   const size_t M = size_t(1) << (logL1 + LOGL + logL3);
@@ -658,24 +661,24 @@ __global__ void __launch_bounds__(NC * (1 << LOGL) / 8)
   const size_t n1 = blockIdx.x / per;
   const int i = threadIdx.x % NC;
   const size_t k3 = size_t(blockIdx.x % per) * NC + i;
-  double2 th[8], tl[8], d[8];
+  double2 th[PT], tl[PT], d[PT];
 #pragma unroll
-  for (int it = 0; it < 8; ++it) {
+  for (int it = 0; it < PT; ++it) {
     const size_t k2 = (threadIdx.x + it * T) / NC;
     pass_root_load(tb, (n1 * k2) << logL3, th[it], tl[it]);
   }
 #pragma unroll
-  for (int it = 0; it < 8; ++it) {
+  for (int it = 0; it < PT; ++it) {
     const size_t n2 = (threadIdx.x + it * T) / NC;
     d[it] = W[((n1 + (n2 << logL1)) << logL3) + k3];
   }
   load_quarter<LOGL, T>(ts, tb.tw);
 #pragma unroll
-  for (int it = 0; it < 8; ++it) z[i * LD + (threadIdx.x + it * T) / NC] = d[it];
+  for (int it = 0; it < PT; ++it) z[i * LD + (threadIdx.x + it * T) / NC] = d[it];
   __syncthreads();
-  fft_cols<LOGL, NC, T, S>(z, ts);
+  fft_cols<LOGL, NC, T, S, PT == 8 ? 3 : 2>(z, ts);
 #pragma unroll
-  for (int it = 0; it < 8; ++it) {
+  for (int it = 0; it < PT; ++it) {
     const size_t k2 = (threadIdx.x + it * T) / NC;
     double2 w = cmul(th[it], tl[it]);
     if constexpr (S > 0) w.y = -w.y;
@@ -751,6 +754,7 @@ struct l2c_dct_plan_s {
   int l_first = 0, c_first = 0, l_mid = 0, l_last = 0, c_last = 0;
   int size = 1;      // elements per work-group in the passes: 0 -> 1024, 1 -> 2048, 2 -> 4096
   int size_mid = 1;  // the same for the middle pass
+  int pt = 8;        // elements per thread in the passes: 8 (radix-8 stages) or 4 (radix 4)
   int dev = 0;
   l2c::Tabs tb{};
   double2 *d_tab = nullptr;   // every twiddle table, back to back
@@ -793,10 +797,10 @@ constexpr int pass_nc() {
   return (1 << LOGL) >= E / 2 ? 2 : E >> LOGL;
 }
 
-template <int LOGL, int E>
+template <int LOGL, int E, int PT>
 void launch_pass_e(const l2c_dct_plan *p, int pass, const double *in, double *out, int kind,
                    hipStream_t st) {
-  constexpr int L = 1 << LOGL, NC = pass_nc<LOGL, E>(), T = NC * L / 8;
+  constexpr int L = 1 << LOGL, NC = pass_nc<LOGL, E>(), T = NC * L / PT;
   const size_t shmem = (size_t(NC) * (L + 1) + L / 4) * sizeof(double2);
   const int other = pass == 1 ? p->c_first : p->c_last;  // number of columns = 2^other
   const dim3 grid(unsigned((size_t(1) << other) / NC), unsigned(p->count));
@@ -804,27 +808,27 @@ void launch_pass_e(const l2c_dct_plan *p, int pass, const double *in, double *ou
   double2 *w1 = p->d_work;
   const double2 *wl = p->l_mid ? p->d_work + p->count * (size_t(1) << p->logM) : p->d_work;
   if (kind == 2 && pass == 1)
-    hipLaunchKernelGGL((k_dct2_p1<LOGL, NC>), grid, dim3(T), shmem, st, in, w1, other, p->tb);
+    hipLaunchKernelGGL((k_dct2_p1<LOGL, NC, PT>), grid, dim3(T), shmem, st, in, w1, other, p->tb);
   else if (kind == 2)
-    hipLaunchKernelGGL((k_dct2_p2<LOGL, NC>), grid, dim3(T), shmem, st, wl, out, other, p->tb);
+    hipLaunchKernelGGL((k_dct2_p2<LOGL, NC, PT>), grid, dim3(T), shmem, st, wl, out, other, p->tb);
   else if (pass == 1)
-    hipLaunchKernelGGL((k_dct3_p1<LOGL, NC>), grid, dim3(T), shmem, st, in, w1, other, p->tb);
+    hipLaunchKernelGGL((k_dct3_p1<LOGL, NC, PT>), grid, dim3(T), shmem, st, in, w1, other, p->tb);
   else
-    hipLaunchKernelGGL((k_dct3_p2<LOGL, NC>), grid, dim3(T), shmem, st, wl, out, other, p->tb);
+    hipLaunchKernelGGL((k_dct3_p2<LOGL, NC, PT>), grid, dim3(T), shmem, st, wl, out, other, p->tb);
 }
 
-template <int LOGL, int E>
+template <int LOGL, int E, int PT>
 void launch_mid_e(const l2c_dct_plan *p, int kind, hipStream_t st) {
-  constexpr int L = 1 << LOGL, NC = E / L, T = E / 8;
+  constexpr int L = 1 << LOGL, NC = E / L, T = E / PT;
   const size_t shmem = (size_t(NC) * (L + 1) + L / 4) * sizeof(double2);
   const dim3 grid(unsigned(size_t(1) << (p->l_last + p->l_first)) / NC, unsigned(p->count));
   const double2 *w1 = p->d_work;
   double2 *w2 = p->d_work + p->count * (size_t(1) << p->logM);
   if (kind == 2)
-    hipLaunchKernelGGL((k_fft_mid<LOGL, NC, -1>), grid, dim3(T), shmem, st, w1, w2, p->l_last,
+    hipLaunchKernelGGL((k_fft_mid<LOGL, NC, -1, PT>), grid, dim3(T), shmem, st, w1, w2, p->l_last,
                        p->l_first, p->tb);
   else
-    hipLaunchKernelGGL((k_fft_mid<LOGL, NC, 1>), grid, dim3(T), shmem, st, w1, w2, p->l_last,
+    hipLaunchKernelGGL((k_fft_mid<LOGL, NC, 1, PT>), grid, dim3(T), shmem, st, w1, w2, p->l_last,
                        p->l_first, p->tb);
 }
 
@@ -837,11 +841,15 @@ template <int LOGL>
 void launch_pass(const l2c_dct_plan *p, int pass, const double *in, double *out, int kind,
                  hipStream_t st) {
   if (p->size == 2)
-    launch_pass_e<LOGL, 2 * kElems>(p, pass, in, out, kind, st);
+    launch_pass_e<LOGL, 2 * kElems, 8>(p, pass, in, out, kind, st);
+  else if (p->size == 1 && p->pt == 4)
+    launch_pass_e<LOGL, kElems, 4>(p, pass, in, out, kind, st);
   else if (p->size == 1)
-    launch_pass_e<LOGL, kElems>(p, pass, in, out, kind, st);
+    launch_pass_e<LOGL, kElems, 8>(p, pass, in, out, kind, st);
+  else if (p->pt == 4)
+    launch_pass_e<LOGL, kElems / 2, 4>(p, pass, in, out, kind, st);
   else
-    launch_pass_e<LOGL, kElems / 2>(p, pass, in, out, kind, st);
+    launch_pass_e<LOGL, kElems / 2, 8>(p, pass, in, out, kind, st);
 }
 
 template <int LOG>
@@ -852,11 +860,15 @@ int dispatch_mid(const l2c_dct_plan *p, int kind, hipStream_t st) {
   } else {
     if (p->l_mid == LOG) {
       if (p->size_mid == 2)
-        launch_mid_e<LOG, 2 * kElems>(p, kind, st);
+        launch_mid_e<LOG, 2 * kElems, 8>(p, kind, st);
+      else if (p->size_mid == 1 && p->pt == 4)
+        launch_mid_e<LOG, kElems, 4>(p, kind, st);
       else if (p->size_mid == 1)
-        launch_mid_e<LOG, kElems>(p, kind, st);
+        launch_mid_e<LOG, kElems, 8>(p, kind, st);
+      else if (p->pt == 4)
+        launch_mid_e<LOG, kElems / 2, 4>(p, kind, st);
       else
-        launch_mid_e<LOG, kElems / 2>(p, kind, st);
+        launch_mid_e<LOG, kElems / 2, 8>(p, kind, st);
       return 0;
     }
     return dispatch_mid<LOG + 1>(p, kind, st);
@@ -1012,6 +1024,12 @@ l2c_dct_plan *create_dct_impl(size_t N, size_t count) {
     p->size = 0;
   // the middle pass's runs of E / L_mid columns must fit in L_first
   p->size_mid = p->size == 2 && p->l_mid + p->l_first < 12 ? 1 : p->size;
+  // passes of length <= 256 run faster with four elements per thread (twice
+  // the waves per work-group; N = 2^13 in 7.7 against 9.0 us, 256 vectors of
+  // 2^13 21.7 against 24.4 us); longer ones with eight (2^20: 23.8 against
+  // 24.3 us).  L2C_DCT_PT = 4 or 8 overrides the rule.
+  p->pt = std::max(p->l_first, std::max(p->l_mid, p->l_last)) <= 8 ? 4 : 8;
+  if (const char *e = std::getenv("L2C_DCT_PT")) p->pt = std::atoi(e) == 4 ? 4 : 8;
   if (hipMalloc((void **)&p->d_tab, total * sizeof(double2)) != hipSuccess ||
       (work && hipMalloc((void **)&p->d_work, work * sizeof(double2)) != hipSuccess) ||
       hipMemcpy(p->d_tab, h.data(), total * sizeof(double2), hipMemcpyHostToDevice) !=

@@ -1941,6 +1941,11 @@ __global__ void __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(4))) k
     role_arrive(ctr + 1);  // done with the windows
   } else {
     RoleSync rs{ctr + 1, nw, 0};
+    // the window phase is address arithmetic and loads: at the default
+    // priority its VALU instructions lose issue to the older work-group's
+    // band FMAs on the same CU, and the second work-group of a CU gets its
+    // windows only when the first one's band is done
+    __builtin_amdgcn_s_setprio(2);
     // ---- the windows: x (both parities), the Lambda span, the head
     const int nX = 2 * ((G + 1) * s + 8);
     constexpr int QX = 9;  // ceil(nX / NT) for G = 16, s = 64
@@ -1995,6 +2000,7 @@ __global__ void __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(4))) k
     // ---- segment A of pair wave - nw (columns [0, s - 8p - shift)), then the
     // C2L diagonal (added after the band: zero-started sums keep a single
     // set of accumulators live through the band's two code paths)
+    __builtin_amdgcn_s_setprio(0);
     const int p = wave - nw;
     double acc[8];
 #pragma unroll
@@ -2084,7 +2090,10 @@ __global__ void __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(4))) k
   double *ob = xs, *ob2 = lw;  // staging after the band (see k_down8)
   auto slot = [&](int e) { return e + (e >> (sh + 1)); };
 
-  // ---- loads: the x window, the Lambda span, the head (one batch)
+  // ---- loads: the x window, the Lambda span, the head (one batch), at
+  // raised priority: their address arithmetic would otherwise wait behind
+  // the band FMAs of an older work-group on the same CU
+  __builtin_amdgcn_s_setprio(2);
   const int nX = 2 * ((G + 1) * s + 8);
   constexpr int QX = 5;  // ceil(nX / NT) for G = 16
   double vx[QX];
@@ -2129,6 +2138,7 @@ __global__ void __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(4))) k
     put_lambda(e, tab[std::min(2 * hc0 + e, (size_t)N - 1)]);
   if (tid < nhd) hd[tid] = vh;
   lds_sync();
+  __builtin_amdgcn_s_setprio(0);
 
   // ---- the band: lane (chunk gl, parity sg, del), one segment per wave
   const int gl = lane & 15, sg = (lane >> 4) & 1, del = lane >> 5;
