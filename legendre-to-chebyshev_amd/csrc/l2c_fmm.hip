@@ -76,7 +76,9 @@ size_t up_shared_bytes(const Geo &g) {
   return 8 * (2 * 16 * (size_t)up_xstride(g.s) + 16 * M2 + 9 * M2 + 2 * (size_t)g.M * Mp);
 }
 
-template <int MT>
+// LM: the subtree levels on MFMA (MT > 0; chosen for subtrees of 2^3 and
+// 2^4 leaf chunks) or on VALU.
+template <int MT, bool LM>
 __global__ void __launch_bounds__(kUpNT) __attribute__((amdgpu_waves_per_eu(4))) k_up(
     const double *__restrict__ in, LineMap lm, Geo g,
                                               int dir, const double *__restrict__ T,
@@ -129,22 +131,39 @@ __global__ void __launch_bounds__(kUpNT) __attribute__((amdgpu_waves_per_eu(4)))
   }
   const int ku = tid % M;
   const int np = g.lu * M2;
-  // U(0), U(1), once per work-group; every load of the batch is in flight
-  // before the first LDS store
-  const int nop = 2 * MM;
-  constexpr int QO = 3;
-  double vo[QO];
+  // A operand of the upward levels on MFMA (LM): rows rt*16 + (lane %
+  // 16) of [U(0) U(1)] (M x 2M), k-steps kk (columns 4 kk + lane / 16), for
+  // waves rt < ntile; loaded from HBM with the input, used from registers
+  constexpr int QA = LM ? (2 * MT + 3) / 4 : 1;
+  double ua[QA];
+  if constexpr (LM) {
+    const int ra = wave * 16 + (lane & 15);
 #pragma unroll
-  for (int q = 0; q < QO; q++) {
-    const int e = q * kUpNT + tid;
-    vo[q] = e < nop ? U[e] : 0.0;
+    for (int kk = 0; kk < QA; kk++) {
+      const int k = 4 * kk + (lane >> 4);
+      const int pu = k < M ? 0 : 1, kc = k - pu * M;
+      ua[kk] = (wave < ntile && ra < M && k < M2) ? U[(size_t)pu * MM + (size_t)ra * M + kc] : 0.0;
+    }
   }
+  // U(0), U(1) in LDS for the VALU levels (M without a compiled kernel),
+  // once per work-group; every load of the batch is in flight before the
+  // first LDS store
+  if constexpr (!LM) {
+    const int nop = 2 * MM;
+    constexpr int QO = 3;
+    double vo[QO];
 #pragma unroll
-  for (int q = 0; q < QO; q++) {
-    const int e = q * kUpNT + tid;
-    if (e < nop) Ul[(e / M) * Mp + e % M] = vo[q];
+    for (int q = 0; q < QO; q++) {
+      const int e = q * kUpNT + tid;
+      vo[q] = e < nop ? U[e] : 0.0;
+    }
+#pragma unroll
+    for (int q = 0; q < QO; q++) {
+      const int e = q * kUpNT + tid;
+      if (e < nop) Ul[(e / M) * Mp + e % M] = vo[q];
+    }
+    for (int e = QO * kUpNT + tid; e < nop; e += kUpNT) Ul[(e / M) * Mp + e % M] = U[e];
   }
-  for (int e = QO * kUpNT + tid; e < nop; e += kUpNT) Ul[(e / M) * Mp + e % M] = U[e];
   for (int e0 = 8 * kUpNT; e0 < nin; e0 += kUpNT) {  // rare: G*2s > 2048
     const int e = e0 + tid;
     if (e < nin) {
@@ -210,11 +229,48 @@ __global__ void __launch_bounds__(kUpNT) __attribute__((amdgpu_waves_per_eu(4)))
   }
   __syncthreads();
 
-  // ---- step 2: parent = U0 child0 + U1 child1 inside the subtree; thread
-  // (k = tid % M) reads its operator rows from LDS
+  // ---- step 2: parent = U0 child0 + U1 child1 inside the subtree.  LM:
+  // on v_mfma_f64_16x16x4f64, parents (M x 2n) = [U0 U1] (M x 2M) times the
+  // children stacked (2M x 2n), column jc = 2c + parity for parent c (2n <=
+  // 16); wave rt < ntile computes row tile rt in two chains of k-steps.
+  // Else on VALU, thread (k = tid % M) reading its operator rows from LDS.
   const int npair = kUpNT / M;  // (chunk, parity) pairs handled per sweep
   int n = G;
-  for (int lev = L - 2; lev >= g.lu; lev--) {
+  if constexpr (LM) {
+    const int jc = lane & 15, cc = jc >> 1, sgc = jc & 1;
+    for (int lev = L - 2; lev >= g.lu; lev--) {
+      n >>= 1;
+      if (wave < ntile) {
+        d4 D0 = {0, 0, 0, 0}, D1 = {0, 0, 0, 0};
+        double bv[QA];
+#pragma unroll
+        for (int kk = 0; kk < QA; kk++) {
+          const int k = 4 * kk + (lane >> 4);
+          const int pu = k < M ? 0 : 1, kc = k - pu * M;
+          bv[kk] = (jc < 2 * n && k < M2) ? w0[((2 * cc + pu) * 2 + sgc) * M + kc] : 0.0;
+        }
+#pragma unroll
+        for (int kk = 0; kk < QA; kk += 2) {
+          D0 = __builtin_amdgcn_mfma_f64_16x16x4f64(ua[kk], bv[kk], D0, 0, 0, 0);
+          if (kk + 1 < QA) D1 = __builtin_amdgcn_mfma_f64_16x16x4f64(ua[kk + 1], bv[kk + 1], D1, 0, 0, 0);
+        }
+        D0 += D1;
+#pragma unroll
+        for (int v = 0; v < 4; v++) {
+          const int i = wave * 16 + (lane >> 4) + 4 * v;
+          if (i < M && jc < 2 * n) {
+            w1[(cc * 2 + sgc) * M + i] = D0[v];
+            w[(slot_base(lev) + (size_t)j * n + cc) * M2 + sgc * M + i] = D0[v];
+          }
+        }
+      }
+      __syncthreads();
+      double *tmp = w0;
+      w0 = w1;
+      w1 = tmp;
+    }
+  }
+  for (int lev = L - 2; !LM && lev >= g.lu; lev--) {
     n >>= 1;
     if (tid < npair * M) {
       for (int cs = tid / M; cs < 2 * n; cs += npair) {
@@ -2962,8 +3018,14 @@ static int launch_fmm_m(fmm_plan_s *p, const Geo &g, const double *d_in, double 
   if (!band_only) {
     dim3 gu((unsigned)((size_t)4 << g.lu), (unsigned)nlines);
     prof_begin(p->prof, K_UP_SUBTREE, st);
-    launch_k(p->prof, k_up<MT>, gu, kUpNT, up_shared_bytes(g), st, d_in, lm, g, (int)dir, p->t.T,
-        p->t.U, p->t.P, ws, wsstride);
+    // subtree levels on MFMA for deep subtrees: 2^20 47.86 -> 47.31 us per
+    // transform at 2^4 chunks; at 2^2 (2^16) the VALU levels are faster
+    if (MT > 0 && g.gu >= 3)
+      launch_k(p->prof, k_up<MT, MT != 0>, gu, kUpNT, up_shared_bytes(g), st, d_in, lm, g, (int)dir,
+               p->t.T, p->t.U, p->t.P, ws, wsstride);
+    else
+      launch_k(p->prof, k_up<MT, false>, gu, kUpNT, up_shared_bytes(g), st, d_in, lm, g, (int)dir,
+               p->t.T, p->t.U, p->t.P, ws, wsstride);
     prof_end(p->prof, st);
     const int ntop = (int)total_blocks(g.lu), ntot = (int)total_blocks(g.L);
     const int bpw = m2l_bpw(g);
