@@ -3018,9 +3018,12 @@ static int launch_fmm_m(fmm_plan_s *p, const Geo &g, const double *d_in, double 
   if (!band_only) {
     dim3 gu((unsigned)((size_t)4 << g.lu), (unsigned)nlines);
     prof_begin(p->prof, K_UP_SUBTREE, st);
-    // subtree levels on MFMA for deep subtrees: 2^20 47.86 -> 47.31 us per
-    // transform at 2^4 chunks; at 2^2 (2^16) the VALU levels are faster
-    if (MT > 0 && g.gu >= 3)
+    // subtree levels on MFMA for deep subtrees of calls of up to 8 lines
+    // (latency-bound: 2^20 47.86 -> 47.31 us per transform, 8 lines of 2^20
+    // 248.5 -> 246.0 us); at 2^2 chunks (2^16) and for calls of 32 lines and
+    // more (throughput-bound: 32 lines of 2^18 234.9 -> 236.8 us) the VALU
+    // levels are as fast or faster
+    if (MT > 0 && g.gu >= 3 && nlines <= 8)
       launch_k(p->prof, k_up<MT, MT != 0>, gu, kUpNT, up_shared_bytes(g), st, d_in, lm, g, (int)dir,
                p->t.T, p->t.U, p->t.P, ws, wsstride);
     else
