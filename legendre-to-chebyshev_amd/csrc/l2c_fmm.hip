@@ -77,7 +77,7 @@ size_t up_shared_bytes(const Geo &g) {
 }
 
 // LM: the subtree levels on MFMA (MT > 0; chosen for subtrees of 2^3 and
-// 2^4 leaf chunks) or on VALU.
+// 2^4 leaf chunks in calls of up to 8 lines) or on VALU.
 template <int MT, bool LM>
 __global__ void __launch_bounds__(kUpNT) __attribute__((amdgpu_waves_per_eu(4))) k_up(
     const double *__restrict__ in, LineMap lm, Geo g,
