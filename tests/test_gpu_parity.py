@@ -265,3 +265,31 @@ def test_mixed_host_device_pointers_rejected(amd, N):
     # matching kinds still work
     u = amd.coefficients(N, seed=3)
     assert relerr(p.execute(u, 0), amd.FMMPlan(N, 64, 18, 0).execute(u, 0)) == 0.0
+
+
+@pytest.mark.parametrize("direction", [0, 1])
+@pytest.mark.parametrize("nlines", [1, 8, 9])
+def test_upward_level_forms(amd, orc, direction, nlines):
+    """k_up's two forms of the subtree levels: on MFMA for subtrees of 2^3 and
+    2^4 leaf chunks in calls of up to 8 lines (N = 2^18: one line, 2^3 chunks;
+    8 lines, 2^4 chunks), on VALU otherwise (9 lines).  Every line of the call
+    is checked against the oracle's single transform (src/C/leg2cheb.c:1385-1399
+    for the upward pass)."""
+    N = 1 << 18
+    announce(f"upward level forms N={N} lines={nlines} dir={direction}")
+    plan = amd.FMMPlan(N, 64, 18, direction)
+    op = orc.FMMPlan(N, 64, 18, direction)
+    u = np.concatenate([amd.coefficients(N, seed=300 + k, decay=1.5 * (k % 2)) for k in range(nlines)])
+    din, dout = amd.DeviceBuffer(u.size), amd.DeviceBuffer(u.size)
+    din.upload(u)
+    if nlines == 1:
+        plan.execute_device(din.ptr, dout.ptr, direction)
+    else:
+        plan.execute_lines_device(din.ptr, dout.ptr, direction, nlines, nlines, N, 0, 1)
+    plan.synchronize()
+    g = dout.download()
+    for k in sorted({0, nlines - 1}):
+        o = op.execute(u[k * N:(k + 1) * N], direction)
+        e = relerr(g[k * N:(k + 1) * N], o)
+        print(f"  line {k}: rel err vs oracle FMM = {e:.3e}")
+        assert e <= tol(N)
