@@ -2440,6 +2440,42 @@ __global__ void __launch_bounds__(256) k_direct(const double *__restrict__ in,
   out[off + (size_t)i * es] = acc;
 }
 
+// k_direct with one wave per output row, for single transforms of larger N
+// (one thread per row leaves a few dozen waves, each a serial loop of N/2
+// terms): the wave's lanes take every 64th term of the row, in the same
+// term order per lane, and add their partial sums across the wave.  Grid
+// ((N + 3) / 4): four rows per work-group.
+__global__ void __launch_bounds__(256) k_direct_rows(const double *__restrict__ in,
+                                                     double *__restrict__ out, LineMap lm, int N,
+                                                     int dir, const double *__restrict__ a,
+                                                     const double *__restrict__ an,
+                                                     const double *__restrict__ dn) {
+  const int i = blockIdx.x * 4 + (threadIdx.x >> 6), lane = threadIdx.x & 63;
+  if (i >= N) return;  // whole waves: the row index is wave-uniform
+  const size_t off = line_off(lm, 0);
+  const double *u = in + off;
+  const size_t es = lm.es;
+  double acc = 0;
+  if (dir == L2C) {
+    for (int n = 2 * lane; n + i < N; n += 128)
+      acc += a[n / 2] * k2Pi * a[n / 2 + i] * u[(size_t)(n + i) * es];
+  } else {
+    for (int n = 2 + 2 * lane; n + i < N; n += 128)
+      acc -= dn[n / 2] * an[n / 2 + i] * (u[(size_t)(n + i) * es] * (double)(n + i));
+  }
+#pragma unroll
+  for (int d = 32; d >= 1; d >>= 1) acc += __shfl_xor(acc, d, 64);
+  if (lane == 0) {
+    if (dir == L2C) {
+      if (i == 0) acc /= 2;
+    } else {
+      const double vi = i == 0 ? u[0] : u[(size_t)i * es] * (double)i;
+      acc = (kSqrtPi * vi * an[i] + acc) * ((double)i + 0.5);
+    }
+    out[off + (size_t)i * es] = acc;
+  }
+}
+
 // ---------------------------------------------------------------------------
 // Batched lines of the direct path (N <= kDenseMaxN) as dense products on FP64
 // MFMA: per parity p, OUT_p (NPP x LT) = K_p (NPP x NPP) IN_p (NPP x LT) for a
@@ -2803,10 +2839,17 @@ size_t gemm_shared_bytes() {
   return 8 * (size_t)kGemmKC * kGemmLTP + kGemmLT * (sizeof(size_t) + sizeof(int));
 }
 
-// k_direct over nlines lines of map lm (one launch for any number of lines)
+// k_direct over nlines lines of map lm (one launch for any number of lines);
+// single transforms from kDirectRowsMinN on take k_direct_rows
+constexpr int kDirectRowsMinN = 1024;
 static void launch_direct(const direct_plan_s *dp, const double *d_in, double *d_out, size_t dir,
                           const LineMap &lm, size_t nlines, hipStream_t st) {
   const int N = (int)dp->N;
+  if (nlines == 1 && N >= kDirectRowsMinN) {
+    launch_k(dp->prof, k_direct_rows, dim3((unsigned)((N + 3) / 4)), 256, 0, st, d_in, d_out, lm,
+             N, (int)dir, dp->d_a, dp->d_an, dp->d_dn);
+    return;
+  }
   const unsigned gy = (unsigned)std::min<size_t>(nlines, dp->lines_y);
   const unsigned gz = (unsigned)((nlines + gy - 1) / gy);
   launch_k(dp->prof, k_direct, dim3((N + 255) / 256, gy, gz), 256, 0, st, d_in, d_out, lm, N,

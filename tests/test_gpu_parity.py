@@ -293,3 +293,34 @@ def test_upward_level_forms(amd, orc, direction, nlines):
         e = relerr(g[k * N:(k + 1) * N], o)
         print(f"  line {k}: rel err vs oracle FMM = {e:.3e}")
         assert e <= tol(N)
+
+
+@pytest.mark.parametrize("N", [1023, 1024, 3001, 8192])
+@pytest.mark.parametrize("direction", [0, 1])
+@pytest.mark.parametrize("stride", [1, 3])
+def test_direct_api_rows(amd, orc, N, direction, stride):
+    """The reference's direct() (src/C/leg2cheb.c:152-234) on a direct plan:
+    below 1024 one thread per row (k_direct), from 1024 on one wave per row
+    (k_direct_rows); strided arrays, both directions, against the oracle's
+    direct sums."""
+    announce(f"direct() N={N} dir={direction} stride={stride}")
+    L = amd.lib()
+    u = amd.coefficients(N, seed=41 + N + direction, decay=1.0 * direction)
+    x = np.zeros(N * stride)
+    x[::stride] = u
+    din, dout = amd.DeviceBuffer(N * stride), amd.DeviceBuffer(N * stride)
+    din.upload(x)
+    dout.upload(np.full(N * stride, 7.0))
+    dp = L.create_direct(N, direction, 0)
+    assert dp
+    try:  # device pointers: the kernels take the stride themselves
+        assert L.direct(din.ptr, dout.ptr, dp, direction, stride) != amd._SIZE_MAX
+    finally:
+        L.free_direct(dp)
+    y = dout.download()
+    o = orc.DirectPlan(N, direction).execute(u, direction)
+    e = relerr(y[::stride], o)
+    print(f"  rel err vs oracle direct = {e:.3e}")
+    assert e <= tol(N)
+    if stride > 1:
+        assert np.all(y.reshape(N, stride)[:, 1:] == 7.0)  # elements between the strided ones untouched
