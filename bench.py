@@ -162,6 +162,42 @@ def copy_calibration(amd, mb):
     del a, b
 
 
+def per_transform_times(plan, din, dout, d, n):
+    """Device time of each of n back-to-back transforms: HIP events recorded
+    before and after every l2c_execute_device call on a stream of its own
+    (the HIP runtime's event API through ctypes).  Returns (median, min) in
+    microseconds."""
+    import ctypes as C
+    hip = C.CDLL("libamdhip64.so")
+    st = C.c_void_p()
+    if hip.hipStreamCreate(C.byref(st)) != 0:
+        raise RuntimeError("hipStreamCreate failed")
+    evs = []
+    try:
+        for _ in range(2 * n):
+            e = C.c_void_p()
+            if hip.hipEventCreate(C.byref(e)) != 0:
+                raise RuntimeError("hipEventCreate failed")
+            evs.append(e)
+        for k in range(n):
+            hip.hipEventRecord(evs[2 * k], st)
+            plan.execute_device(din.ptr, dout.ptr, d, stream=st.value)
+            hip.hipEventRecord(evs[2 * k + 1], st)
+        if hip.hipEventSynchronize(evs[-1]) != 0:
+            raise RuntimeError("hipEventSynchronize failed")
+        us = []
+        for k in range(n):
+            ms = C.c_float()
+            hip.hipEventElapsedTime(C.byref(ms), evs[2 * k], evs[2 * k + 1])
+            us.append(1e3 * ms.value)
+    finally:
+        for e in evs:
+            hip.hipEventDestroy(e)
+        hip.hipStreamDestroy(st)
+    us.sort()
+    return us[len(us) // 2], us[0]
+
+
 def bench_batch(amd, orc, args):
     """The batched half of the metric: `--batch` transforms of uneven length
     n = floor(exp(U(ln 64, ln 8193))) (SplitMix64 seed 2024), inputs
@@ -534,6 +570,9 @@ def main():
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--check-rows", type=int, default=512,
                     help="rows checked against the double-double direct sum")
+    ap.add_argument("--per-transform", type=int, default=200,
+                    help="transforms timed one by one with HIP events for the median and min "
+                         "device time (0 = skip)")
     ap.add_argument("--prof-every", type=int, default=1,
                     help="time the kernels with HIP events in a separate pass after the timed "
                          "steps (0 = skip)")
@@ -716,7 +755,15 @@ def main():
         prof = plan.profile_read()
         plan.profile(False)
         kernels = {k: {"avg_us": 1e3 * ms / max(c, 1), "launches": c} for k, (ms, c) in prof.items()}
-    log(f"[bench] {value:.1f} transforms/s, {ms_per_step * 1e3:.2f} us/step; kernels {kernels}")
+    # per-transform device times (median and min over separately bracketed
+    # transforms, outside the timed loop)
+    per = None
+    if args.per_transform > 0:
+        med, mn = per_transform_times(plan, din, dout, d, args.per_transform)
+        per = {"median_us": round(med, 3), "min_us": round(mn, 3), "samples": args.per_transform,
+               "timing": "HIP events before and after each l2c_execute_device call"}
+    log(f"[bench] {value:.1f} transforms/s, {ms_per_step * 1e3:.2f} us/step; kernels {kernels}; "
+        f"per transform {per}")
 
     # correctness against the oracle
     g = dout.download()
@@ -1002,6 +1049,7 @@ def main():
         "max_rel_err": {"vs_oracle_fmm": err_fmm, "vs_dd_direct_rows": err_direct,
                         "rows_checked": int(rows.size), "tol": tol},
         "kernels_us": {k: round(v["avg_us"], 3) for k, v in kernels.items()},
+        "per_transform": per,
         "roofline": roofline,
         "cold_cache": cold,
         "cpu_baseline": cpu,
