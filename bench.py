@@ -568,7 +568,7 @@ def main():
     ap.add_argument("--cpu-seconds", type=float, default=10.0,
                     help="bounded CPU-baseline sample (seconds of oracle work)")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
-    ap.add_argument("--check-rows", type=int, default=512,
+    ap.add_argument("--check-rows", type=int, default=4352,
                     help="rows checked against the double-double direct sum")
     ap.add_argument("--per-transform", type=int, default=200,
                     help="transforms timed one by one with HIP events for the median and min "
