@@ -757,13 +757,13 @@ def main():
         kernels = {k: {"avg_us": 1e3 * ms / max(c, 1), "launches": c} for k, (ms, c) in prof.items()}
     # per-transform device times (median and min over separately bracketed
     # transforms, outside the timed loop)
-    per = None
+    per_tx = None
     if args.per_transform > 0:
         med, mn = per_transform_times(plan, din, dout, d, args.per_transform)
-        per = {"median_us": round(med, 3), "min_us": round(mn, 3), "samples": args.per_transform,
-               "timing": "HIP events before and after each l2c_execute_device call"}
+        per_tx = {"median_us": round(med, 3), "min_us": round(mn, 3), "samples": args.per_transform,
+                  "timing": "HIP events before and after each l2c_execute_device call"}
     log(f"[bench] {value:.1f} transforms/s, {ms_per_step * 1e3:.2f} us/step; kernels {kernels}; "
-        f"per transform {per}")
+        f"per transform {per_tx}")
 
     # correctness against the oracle
     g = dout.download()
@@ -1049,7 +1049,7 @@ def main():
         "max_rel_err": {"vs_oracle_fmm": err_fmm, "vs_dd_direct_rows": err_direct,
                         "rows_checked": int(rows.size), "tol": tol},
         "kernels_us": {k: round(v["avg_us"], 3) for k, v in kernels.items()},
-        "per_transform": per,
+        "per_transform": per_tx,
         "roofline": roofline,
         "cold_cache": cold,
         "cpu_baseline": cpu,
