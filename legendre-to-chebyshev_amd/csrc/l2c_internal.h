@@ -270,8 +270,10 @@ struct fmm_plan_s {
   // FP64 recombination)
   int precision = 0;
   // M2L of levels >= lu on FP64 MFMA for calls of at least this many lines
-  // (0: never); L2C_M2L_MMA sets it at plan creation
-  int m2l_mma_lines = 0;
+  // (0: never; from 32 lines on the MFMA form is faster: the uneven batch of
+  // 4096 transforms 14.0 -> 14.9 M/s, while 8 lines of 2^20 lose 244 -> 259
+  // us); L2C_M2L_MMA sets it at plan creation
+  int m2l_mma_lines = 32;
   // downward pass of s = 16/32/64 plans: 0 (default) one fused kernel, 1
   // k_band + k_far, 2 k_band + k_far for calls of several lines with L >= 10;
   // L2C_DOWN_SPLIT

@@ -5,9 +5,9 @@
   anti-diagonal, FP64 recombination); the error against the oracle FMM and
   against double-double direct-sum rows is stated per N and must be at most
   1e-13 (the precision form's bound);
-- the FP64 MFMA form of the same products for line batches
-  (L2C_M2L_MMA=<min lines> at plan creation): equal to the VALU M2L within
-  1e-15 relative.
+- the FP64 MFMA form of the same products for line batches (by default for
+  calls of 32 lines and more; L2C_M2L_MMA=<min lines> at plan creation, 0
+  off): equal to the VALU M2L within 1e-15 relative.
 """
 import math
 
@@ -73,8 +73,10 @@ def test_form2_lines(amd, orc, N, lines):
 
 @pytest.mark.parametrize("N,lines", [(1 << 16, 20), (1 << 20, 3), (4096, 33)])
 def test_m2l_mma_fp64_lines(amd, monkeypatch, N, lines):
-    """The FP64 MFMA M2L for line batches gives the VALU M2L's result."""
+    """The FP64 MFMA M2L for line batches (the default from 32 lines on)
+    gives the VALU M2L's result."""
     announce(f"m2l mma fp64 N={N} lines={lines}")
+    monkeypatch.setenv("L2C_M2L_MMA", "0")
     p0 = amd.FMMPlan(N, 64, 18, amd.BOTH)
     monkeypatch.setenv("L2C_M2L_MMA", "1")
     p1 = amd.FMMPlan(N, 64, 18, amd.BOTH)
