@@ -60,6 +60,8 @@ def kernel_bytes(name, info, N):
     k_down: input, output, the Lambda table the band reads (a for L2C, an for
     C2L; the 2s-entry head is negligible) and one M2L vector per chunk slot.
     k_up: input, one multipole vector per chunk slot.
+    k_direct (single transforms of N <= 8192 on an FMM plan): input, output
+    and the Lambda table a (C2L reads an and dn, 1.5 N; counted as N).
     """
     M, L, Nb, Nc = info["M"], info["L"], info["Nb"], info["Nc"]
     slots = (4 << L) - 4
@@ -69,6 +71,8 @@ def kernel_bytes(name, info, N):
         return 8 * (3 * N + slots * 2 * M)
     if name == "k_up":
         return 8 * (N + slots * 2 * M)
+    if name == "k_direct":
+        return 8 * 3 * N
     return 8 * 2 * N
 
 

@@ -2455,6 +2455,9 @@ __global__ void __launch_bounds__(256) k_direct_rows(const double *__restrict__ 
   const size_t off = line_off(lm, 0);
   const double *u = in + off;
   const size_t es = lm.es;
+  // (four independent sums per lane, so that a trip's loads are in flight
+  // together, measured no faster: 10.3 against 9.6 us at N = 4096; the
+  // kernel is bound by its three loads per term, not by their latency)
   double acc = 0;
   if (dir == L2C) {
     for (int n = 2 * lane; n + i < N; n += 128)
@@ -2473,6 +2476,159 @@ __global__ void __launch_bounds__(256) k_direct_rows(const double *__restrict__ 
       acc = (kSqrtPi * vi * an[i] + acc) * ((double)i + 0.5);
     }
     out[off + (size_t)i * es] = acc;
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Direct sums of a single transform of small N with the band's register
+// rings (src/C/leg2cheb.c:152-234).  Per parity sg the rows r and columns m
+// (element 2m + sg) run over [0, R), R = (N - sg + 1) / 2, and
+//   y_r = sum_{m >= r} H(m - r) P(m + r + sg) X(m)
+// with H = P = Lambda, X = u (L2C), or H = dn (H(0) = 0: the diagonal is
+// added after the sum), P = an, X(m) = u_n n (C2L).  A work-group owns 32
+// rows of one parity (four groups of 8 rows, one per wave) and stages X, P
+// and H for every column its rows touch in LDS; lane j of a wave takes the
+// columns t = m - r0 in [jT, jT + T) of its group's 8 rows, T a multiple of 8
+// with T/8 odd, and the 64 lanes' sums are added across the wave.  Every
+// window stores index e at e + e/8, so the 64 lanes' addresses (64 segments
+// of T/8 padded blocks apart) fall in distinct banks.
+constexpr int kDirectBandMinN = 3072, kDirectBandMaxN = 8192;
+__host__ __device__ inline int dband_win(int R) { return R + 128; }
+__host__ __device__ inline int dband_pad(int e) { return e + (e >> 3); }
+size_t dband_shared_bytes(int N) {
+  const int w = dband_pad(dband_win((N + 1) / 2)) + 1;
+  return 8 * (size_t)(3 * w);
+}
+
+// Columns [t0, t1) of the lane's 8 rows, t0 a multiple of 8; xq, pq, hq: the
+// padded blocks of X(r0 + t0), P(2 r0 + sg + t0), H(t0).  Columns t >= len
+// read X as zero.  Rings as in band_seg8: slot t mod 8 holds H(t - 8 .. t)
+// and P(t .. t + 7); row rho of column t takes H(t - rho) P(t + rho) X(t).
+__device__ __forceinline__ void direct_seg8(const double *__restrict__ xq,
+                                            const double *__restrict__ pq,
+                                            const double *__restrict__ hq, int t0, int t1,
+                                            int len, double acc[8]) {
+  const int ntrip = t1 > t0 ? (t1 - t0 + 7) >> 3 : 0;
+  if (ntrip == 0) return;
+  double hr[8], pr[8];
+#pragma unroll
+  for (int k = 0; k < 8; k++) {
+    hr[k] = t0 >= 8 ? hq[k - 9] : 0.0;
+    pr[k] = pq[k];
+  }
+  int tt = t0;
+  double XA[4], PA[4], HA[4], XB[4], PB[4], HB[4];
+#define DFETCH(X, P, H, HALF)                                       \
+  _Pragma("unroll") for (int k_ = 0; k_ < 4; k_++) {                \
+    X[k_] = tt + 4 * (HALF) + k_ < len ? xq[4 * (HALF) + k_] : 0.0; \
+    P[k_] = pq[9 + 4 * (HALF) + k_];                                \
+    H[k_] = hq[4 * (HALF) + k_];                                    \
+  }
+#define DSTEP(K, X, P, H, J)                                          \
+  {                                                                   \
+    hr[K] = H[J];                                                     \
+    const double x_ = X[J];                                           \
+    _Pragma("unroll") for (int r_ = 0; r_ < 8; r_++)                  \
+      acc[r_] += (hr[((K) - r_) & 7] * pr[((K) + r_) & 7]) * x_;      \
+    pr[K] = P[J];                                                     \
+  }
+  DFETCH(XA, PA, HA, 0)
+  for (int b = 0; b < ntrip; b++) {
+    DFETCH(XB, PB, HB, 1)
+    DSTEP(0, XA, PA, HA, 0) DSTEP(1, XA, PA, HA, 1) DSTEP(2, XA, PA, HA, 2) DSTEP(3, XA, PA, HA, 3)
+    xq += 9;
+    pq += 9;
+    hq += 9;
+    tt += 8;
+    DFETCH(XA, PA, HA, 0)
+    DSTEP(4, XB, PB, HB, 0) DSTEP(5, XB, PB, HB, 1) DSTEP(6, XB, PB, HB, 2) DSTEP(7, XB, PB, HB, 3)
+  }
+#undef DSTEP
+#undef DFETCH
+}
+
+// Grid ((R0 + 31) / 32, 2) with R0 = (N + 1) / 2, 256 threads,
+// dband_shared_bytes(N) of LDS.
+template <int DIR>
+__global__ void __launch_bounds__(256) k_direct_band(const double *__restrict__ in,
+                                                     double *__restrict__ out, LineMap lm, int N,
+                                                     const double *__restrict__ a,
+                                                     const double *__restrict__ an,
+                                                     const double *__restrict__ dn) {
+  extern __shared__ double sm[];
+  const int sg = blockIdx.y;
+  const int R = (N - sg + 1) / 2;
+  const int Rb = blockIdx.x * 32;
+  if (Rb >= R) return;  // the whole work-group
+  const int tid = threadIdx.x, lane = tid & 63;
+  const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
+  const int span = R - Rb;  // columns Rb .. R - 1
+  const int W = dband_win(span), WP = dband_pad(W) + 1;
+  double *xw = sm, *pw = sm + WP, *hw = sm + 2 * WP;
+  const size_t off = line_off(lm, 0);
+  const double *u = in + off;
+  const size_t es = lm.es;
+  const double *Ht = DIR == 0 ? a : dn;
+  const double *Pt = DIR == 0 ? a : an;
+  const int nH = DIR == 0 ? N : (N + 1) / 2;  // entries of the H table
+  // the windows in batches of QF entries per thread, the batch's loads
+  // issued before its stores (one entry per trip: 8.6 against 8.2 us at
+  // N = 4096; QF = 16 takes 174 VGPRs and was slower still)
+  constexpr int QF = 4;
+  for (int e0 = tid; e0 < W; e0 += QF * 256) {
+    double vx[QF], vp[QF], vh[QF];
+#pragma unroll
+    for (int k = 0; k < QF; k++) {
+      const int e = e0 + k * 256;
+      const int m = Rb + e, qi = 2 * Rb + sg + e;
+      vx[k] = m < R ? u[(size_t)(2 * m + sg) * es] : 0.0;
+      vp[k] = qi < N ? Pt[qi] : 0.0;
+      vh[k] = (e < span && e < nH && !(DIR == 1 && e == 0)) ? Ht[e] : 0.0;
+    }
+#pragma unroll
+    for (int k = 0; k < QF; k++) {
+      const int e = e0 + k * 256;
+      if (e < W) {
+        const int pe = dband_pad(e);
+        xw[pe] = DIR == 1 ? vx[k] * (double)(2 * (Rb + e) + sg) : vx[k];
+        pw[pe] = vp[k];
+        hw[pe] = vh[k];
+      }
+    }
+  }
+  __syncthreads();
+  const int r0 = Rb + 8 * wave;
+  if (r0 >= R) return;  // no barrier below
+  const int len = R - r0;
+  int T8 = (span + 511) / 512;
+  T8 |= 1;
+  const int T = 8 * T8;
+  const int t0 = lane * T, t1 = std::min(t0 + T, len);
+  double acc[8];
+#pragma unroll
+  for (int k = 0; k < 8; k++) acc[k] = 0.0;
+  if (t0 < len)
+    direct_seg8(xw + dband_pad(8 * wave + t0), pw + dband_pad(16 * wave + t0), hw + dband_pad(t0),
+                t0, t1, len, acc);
+#pragma unroll
+  for (int k = 0; k < 8; k++) {
+#pragma unroll
+    for (int d = 32; d >= 1; d >>= 1) acc[k] += __shfl_xor(acc[k], d, 64);
+  }
+  double v = acc[0];
+#pragma unroll
+  for (int k = 1; k < 8; k++) v = lane == k ? acc[k] : v;
+  const int r = r0 + lane;
+  if (lane < 8 && r < R) {
+    const int i = 2 * r + sg;
+    if (DIR == 0) {
+      v *= k2Pi;
+      if (i == 0) v *= 0.5;
+    } else {
+      const double vi = i == 0 ? u[0] : u[(size_t)i * es] * (double)i;
+      v = (kSqrtPi * vi * an[i] - v) * ((double)i + 0.5);
+    }
+    out[off + (size_t)i * es] = v;
   }
 }
 
@@ -2839,15 +2995,39 @@ size_t gemm_shared_bytes() {
   return 8 * (size_t)kGemmKC * kGemmLTP + kGemmLT * (sizeof(size_t) + sizeof(int));
 }
 
+// One transform by direct sums from the Lambda tables a, an, dn (those of a
+// direct plan of size N, or of an FMM plan): k_direct_band for N in
+// (kDirectBandMinN, kDirectBandMaxN] unless band is false (the plan's
+// direct_band), else k_direct_rows.  Kernel trace, L2C, k_direct_band /
+// k_direct_rows: N = 1024 5.8 / 3.8 us, 4096 8.2 / 9.7 us, 8192 12.2 /
+// 27.8 us (below about 3000 the band kernel's window loads and cross-lane
+// sums cost more than the row kernel's loads save).
+static void launch_direct_one(Prof *prof, int N, size_t dir, const double *a, const double *an,
+                              const double *dn, const double *d_in, double *d_out,
+                              const LineMap &lm, hipStream_t st, bool band) {
+  if (N > kDirectBandMinN && N <= kDirectBandMaxN && band) {
+    const dim3 grid((unsigned)(((N + 1) / 2 + 31) / 32), 2);
+    if (dir == L2C)
+      launch_k(prof, k_direct_band<0>, grid, 256, dband_shared_bytes(N), st, d_in, d_out, lm, N,
+               a, an, dn);
+    else
+      launch_k(prof, k_direct_band<1>, grid, 256, dband_shared_bytes(N), st, d_in, d_out, lm, N,
+               a, an, dn);
+    return;
+  }
+  launch_k(prof, k_direct_rows, dim3((unsigned)((N + 3) / 4)), 256, 0, st, d_in, d_out, lm, N,
+           (int)dir, a, an, dn);
+}
+
 // k_direct over nlines lines of map lm (one launch for any number of lines);
-// single transforms from kDirectRowsMinN on take k_direct_rows
+// single transforms from kDirectRowsMinN on take launch_direct_one
 constexpr int kDirectRowsMinN = 1024;
 static void launch_direct(const direct_plan_s *dp, const double *d_in, double *d_out, size_t dir,
                           const LineMap &lm, size_t nlines, hipStream_t st) {
   const int N = (int)dp->N;
   if (nlines == 1 && N >= kDirectRowsMinN) {
-    launch_k(dp->prof, k_direct_rows, dim3((unsigned)((N + 3) / 4)), 256, 0, st, d_in, d_out, lm,
-             N, (int)dir, dp->d_a, dp->d_an, dp->d_dn);
+    launch_direct_one(dp->prof, N, dir, dp->d_a, dp->d_an, dp->d_dn, d_in, d_out, lm, st,
+                      dp->direct_band);
     return;
   }
   const unsigned gy = (unsigned)std::min<size_t>(nlines, dp->lines_y);
@@ -3153,6 +3333,16 @@ static int launch_fmm(fmm_plan_s *p, const double *d_in, double *d_out, size_t d
 int run_fmm_launch(fmm_plan_s *p, const double *d_in, double *d_out, size_t dir, size_t stride,
                    hipStream_t st, int band_only) {
   LineMap lm{1, 0, 0, stride};
+  if (!band_only && p->N <= (size_t)p->direct_max_n) {
+    // small N: the direct sums in one launch (the plan's tables a, an, dn
+    // are those of a direct plan of the same N)
+    prof_begin(p->prof, K_DIRECT, st);
+    launch_direct_one(p->prof, (int)p->N, dir, p->t.a, p->t.an, p->t.dn, d_in, d_out, lm, st,
+                      p->direct_band);
+    prof_end(p->prof, st);
+    L2C_HIP(hipGetLastError(), -1);
+    return 0;
+  }
   return launch_fmm(p, d_in, d_out, dir, lm, 1, st, band_only);
 }
 int run_fmm(fmm_plan_s *p, const double *d_in, double *d_out, size_t dir, size_t stride,

@@ -255,6 +255,9 @@ struct direct_plan_s {
   // lines per grid row of k_direct (grid.y limit 65535; L2C_DIRECT_LINES_Y
   // sets a smaller value, which the tests use to reach grid.z > 1 cheaply)
   unsigned lines_y = 65535;
+  // single transforms up to N = 8192 on k_direct_band, else k_direct_rows
+  // (L2C_DIRECT_BAND=0: always k_direct_rows)
+  bool direct_band = true;
   double *d_in = nullptr, *d_out = nullptr;  // staging (device)
   double *h_in = nullptr, *h_out = nullptr;  // staging (pinned host)
   hipStream_t stream = nullptr;
@@ -285,6 +288,15 @@ struct fmm_plan_s {
   // band columns moved from k_down8r's band role to its tree role (negative:
   // the other way); L2C_ROLES_SHIFT
   int roles_shift = -32;
+  // single transforms (execute, l2c_execute_device) of N up to this run as
+  // one direct O(N^2) launch (k_direct_rows) on the plan's Lambda tables
+  // instead of the three FMM launches, whose dependent chains set a floor of
+  // about 22 us whatever the size; 0: always the FMM; L2C_DIRECT_MAXN
+  int direct_max_n = 8192;
+  // single direct transforms up to N = 8192 on k_direct_band (the band's
+  // register rings over LDS windows) rather than k_direct_rows (three loads
+  // per term); L2C_DIRECT_BAND=0 selects k_direct_rows
+  bool direct_band = true;
   // work-group geometry of line calls (geo) and of single-line calls (geo1):
   // the same tree, cut into subtrees of different size; the composite
   // operators P and Q hold levels 1..lu_tab and 1..ld_tab, enough for both

@@ -453,6 +453,7 @@ direct_plan_s *make_direct(size_t N, size_t direction, size_t precompute) {
     const long v = atol(e);
     if (v >= 1 && v <= 65535) dp->lines_y = (unsigned)v;
   }
+  if (const char *e = getenv("L2C_DIRECT_BAND")) dp->direct_band = atoi(e) != 0;
   (void)hipGetDevice(&dp->device);
   if (hipStreamCreateWithFlags(&dp->stream, hipStreamNonBlocking) != hipSuccess) {
     set_error("hipStreamCreate failed");
@@ -602,6 +603,8 @@ fmm_plan_s *create_fmm_impl(size_t N, size_t maxs, size_t M, size_t direction, s
   if (const char *e = getenv("L2C_M2L_MMA")) p->m2l_mma_lines = std::max(0, atoi(e));
   if (const char *e = getenv("L2C_DOWN_ROLES")) p->down_roles = atoi(e) != 0;
   if (const char *e = getenv("L2C_ROLES_SHIFT")) p->roles_shift = atoi(e);
+  if (const char *e = getenv("L2C_DIRECT_MAXN")) p->direct_max_n = std::max(0, atoi(e));
+  if (const char *e = getenv("L2C_DIRECT_BAND")) p->direct_band = atoi(e) != 0;
   (void)hipGetDevice(&p->device);
   if (hipStreamCreateWithFlags(&p->stream, hipStreamNonBlocking) != hipSuccess) {
     set_error("hipStreamCreate failed");

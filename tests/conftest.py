@@ -4,6 +4,12 @@ import sys
 
 import pytest
 
+# FMM plans of N <= 8192 run single transforms as direct sums by default
+# (plan field direct_max_n); the suite's FMM cases at those sizes are there
+# to test the FMM kernels, so plans made under it keep the FMM unless a test
+# sets L2C_DIRECT_MAXN itself (tests/test_gpu_small_n.py)
+os.environ.setdefault("L2C_DIRECT_MAXN", "0")
+
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 PKG_DIR = os.path.join(ROOT, "legendre-to-chebyshev_amd")
 if ROOT not in sys.path:

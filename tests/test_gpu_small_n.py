@@ -1,0 +1,134 @@
+"""Single transforms of small N through an FMM plan: up to N = 8192 (the
+plan field direct_max_n, L2C_DIRECT_MAXN) execute() and l2c_execute_device
+run the direct sums of src/C/leg2cheb.c:152-234 in one launch
+(k_direct_rows, or k_direct_band from N = 3073) instead of the three FMM
+launches.  Checked against the oracle's FMM (the reference's execute) and the
+double-double direct sums, tolerance 1e-14 log2 N, and against the plan's own
+FMM route."""
+import math
+
+import numpy as np
+import pytest
+
+from conftest import announce, relerr
+
+pytestmark = pytest.mark.gpu
+
+
+def tol(N):
+    return 1e-14 * math.log2(max(N, 2))
+
+
+SIZES = [1000, 1024, 3001, 3072, 3073, 4096, 6000, 8191, 8192]
+
+
+def make_plan(amd, monkeypatch, N, maxs, direction, maxn=None, band=None):
+    """A plan under the default route (maxn None) or L2C_DIRECT_MAXN=maxn."""
+    if maxn is None:
+        monkeypatch.delenv("L2C_DIRECT_MAXN", raising=False)
+    else:
+        monkeypatch.setenv("L2C_DIRECT_MAXN", str(maxn))
+    if band is None:
+        monkeypatch.delenv("L2C_DIRECT_BAND", raising=False)
+    else:
+        monkeypatch.setenv("L2C_DIRECT_BAND", str(band))
+    return amd.FMMPlan(N, maxs, 18, direction)
+
+
+def route(plan, run):
+    """The profile slots one call of run() fills."""
+    plan.profile(True)
+    run()
+    slots = set(plan.profile_read())
+    plan.profile(False)
+    return slots
+
+
+@pytest.mark.parametrize("N", SIZES)
+@pytest.mark.parametrize("direction", [0, 1])
+@pytest.mark.parametrize("decay", [0.0, 1.5])
+def test_small_n_default_route(amd, orc, monkeypatch, N, direction, decay):
+    announce(f"small-N route N={N} dir={direction} decay={decay}")
+    u = amd.coefficients(N, seed=500 + N, decay=decay)
+    p = make_plan(amd, monkeypatch, N, 64, direction)
+    assert route(p, lambda: p.execute(u, direction)) == {"k_direct"}
+    g = p.execute(u, direction)
+    o = orc.FMMPlan(N, 64, 18, direction).execute(u, direction)
+    e = relerr(g, o)
+    ref = orc.leg2cheb_dd(u) if direction == 0 else orc.cheb2leg_dd(u)
+    e2 = relerr(g, ref)
+    print(f"  rel err vs oracle FMM {e:.3e}, vs double-double direct {e2:.3e}")
+    assert e <= tol(N) and e2 <= tol(N)
+    # the same plan size on the FMM route agrees within the same bound
+    q = make_plan(amd, monkeypatch, N, 64, direction, maxn=0)
+    assert "k_up" in route(q, lambda: q.execute(u, direction))
+    assert relerr(g, q.execute(u, direction)) <= tol(N)
+
+
+@pytest.mark.parametrize("N", [3073, 4096, 8192])
+@pytest.mark.parametrize("direction", [0, 1])
+def test_small_n_kernels_agree(amd, monkeypatch, N, direction):
+    """k_direct_band against k_direct_rows (L2C_DIRECT_BAND=0) on the same
+    plan size: the same sums in another order."""
+    announce(f"direct kernels N={N} dir={direction}")
+    u = amd.coefficients(N, seed=77 + N, decay=1.0)
+    a = make_plan(amd, monkeypatch, N, 64, direction).execute(u, direction)
+    b = make_plan(amd, monkeypatch, N, 64, direction, band=0).execute(u, direction)
+    e = relerr(a, b)
+    print(f"  band vs rows {e:.3e}")
+    assert e <= 2e-15
+
+
+@pytest.mark.parametrize("N", [4096, 8191])
+@pytest.mark.parametrize("direction", [0, 1])
+@pytest.mark.parametrize("graph", [False, True])
+def test_small_n_device_strided(amd, orc, monkeypatch, N, direction, graph):
+    """l2c_execute_device with stride 3, plain and replayed from the plan's
+    HIP graph (three replays, each checked); elements between the strided
+    ones stay untouched."""
+    announce(f"small-N device N={N} dir={direction} graph={graph}")
+    stride = 3
+    u = amd.coefficients(N, seed=9 + N + direction)
+    x = np.zeros(N * stride)
+    x[::stride] = u
+    din, dout = amd.DeviceBuffer(N * stride), amd.DeviceBuffer(N * stride)
+    din.upload(x)
+    p = make_plan(amd, monkeypatch, N, 64, direction)
+    if graph:
+        p.graph(True)
+    o = orc.FMMPlan(N, 64, 18, direction).execute(u, direction)
+    for rep in range(3 if graph else 1):
+        dout.upload(np.full(N * stride, 7.0))
+        p.execute_device(din.ptr, dout.ptr, direction, stride)
+        p.synchronize()
+        y = dout.download()
+        e = relerr(y[::stride], o)
+        print(f"  replay {rep}: rel err vs oracle FMM {e:.3e}")
+        assert e <= tol(N)
+        assert np.all(y.reshape(N, stride)[:, 1:] == 7.0)
+
+
+@pytest.mark.parametrize("N", [8193, 16384])
+def test_above_threshold_keeps_fmm(amd, orc, monkeypatch, N):
+    """Past 8192 the FMM runs as before."""
+    announce(f"above threshold N={N}")
+    u = amd.coefficients(N, seed=3)
+    p = make_plan(amd, monkeypatch, N, 64, 0)
+    assert "k_up" in route(p, lambda: p.execute(u, 0))
+    assert relerr(p.execute(u, 0), orc.FMMPlan(N, 64, 18, 0).execute(u, 0)) <= tol(N)
+
+
+def test_lines_keep_fmm(amd, monkeypatch):
+    """Line calls (nlines > 1) of a small plan stay on the FMM."""
+    announce("small-N lines stay FMM")
+    N, B = 4096, 4
+    p = make_plan(amd, monkeypatch, N, 64, 0)
+    u = np.concatenate([amd.coefficients(N, seed=60 + b) for b in range(B)])
+    din, dout = amd.DeviceBuffer(B * N), amd.DeviceBuffer(B * N)
+    din.upload(u)
+    slots = route(p, lambda: (p.execute_lines_device(din.ptr, dout.ptr, 0, B, B, N, 0, 1),
+                              p.synchronize()))
+    assert "k_up" in slots and "k_direct" not in slots
+    y = dout.download().reshape(B, N)
+    for b in range(B):
+        assert relerr(y[b], p.execute(u[b * N:(b + 1) * N], 0)) <= tol(N)
