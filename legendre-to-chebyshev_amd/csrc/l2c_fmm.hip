@@ -2485,19 +2485,25 @@ __global__ void __launch_bounds__(256) k_direct_rows(const double *__restrict__ 
 // (element 2m + sg) run over [0, R), R = (N - sg + 1) / 2, and
 //   y_r = sum_{m >= r} H(m - r) P(m + r + sg) X(m)
 // with H = P = Lambda, X = u (L2C), or H = dn (H(0) = 0: the diagonal is
-// added after the sum), P = an, X(m) = u_n n (C2L).  A work-group owns 32
-// rows of one parity (four groups of 8 rows, one per wave) and stages X, P
-// and H for every column its rows touch in LDS; lane j of a wave takes the
-// columns t = m - r0 in [jT, jT + T) of its group's 8 rows, T a multiple of 8
-// with T/8 odd, and the 64 lanes' sums are added across the wave.  Every
-// window stores index e at e + e/8, so the 64 lanes' addresses (64 segments
-// of T/8 padded blocks apart) fall in distinct banks.
+// added after the sum), P = an, X(m) = u_n n (C2L).  A work-group of four
+// waves owns 8 G rows of one parity (G groups of 8 rows, 4 / G waves each)
+// and stages X, P and H for every column its rows touch in LDS; lane j of
+// the wave for part c of group g takes the columns t = m - r0 in
+// [(64c + j) T, (64c + j + 1) T) of the group's 8 rows, T a multiple of 8
+// with T/8 odd; the 64 lanes' sums are added across the wave, then the
+// parts through LDS.  Every window stores index e at e + e/8, so the 64
+// lanes' addresses (64 segments of T/8 padded blocks apart) fall in distinct
+// banks.  G = 2 up to N = 6144 (kernel trace at 4096: 7.3 against 8.0 us
+// with G = 4), G = 4 above (8192: 11.8 against 13.0 us: the windows, loaded
+// once per work-group, then cost more than the shorter column runs save).
 constexpr int kDirectBandMinN = 3072, kDirectBandMaxN = 8192;
 __host__ __device__ inline int dband_win(int R) { return R + 128; }
 __host__ __device__ inline int dband_pad(int e) { return e + (e >> 3); }
+constexpr int kDbSplitMaxN = 6144;  // G = 2 up to this N, else 4
+inline int dband_groups(int N) { return N <= kDbSplitMaxN ? 2 : 4; }
 size_t dband_shared_bytes(int N) {
   const int w = dband_pad(dband_win((N + 1) / 2)) + 1;
-  return 8 * (size_t)(3 * w);
+  return 8 * (size_t)(3 * w + 16);  // windows, the second parts' row sums (G = 2)
 }
 
 // Columns [t0, t1) of the lane's 8 rows, t0 a multiple of 8; xq, pq, hq: the
@@ -2547,9 +2553,9 @@ __device__ __forceinline__ void direct_seg8(const double *__restrict__ xq,
 #undef DFETCH
 }
 
-// Grid ((R0 + 31) / 32, 2) with R0 = (N + 1) / 2, 256 threads,
+// Grid ((R0 + 8 G - 1) / (8 G), 2) with R0 = (N + 1) / 2, 256 threads,
 // dband_shared_bytes(N) of LDS.
-template <int DIR>
+template <int DIR, int G>
 __global__ void __launch_bounds__(256) k_direct_band(const double *__restrict__ in,
                                                      double *__restrict__ out, LineMap lm, int N,
                                                      const double *__restrict__ a,
@@ -2558,7 +2564,9 @@ __global__ void __launch_bounds__(256) k_direct_band(const double *__restrict__ 
   extern __shared__ double sm[];
   const int sg = blockIdx.y;
   const int R = (N - sg + 1) / 2;
-  const int Rb = blockIdx.x * 32;
+  static_assert(G == 2 || G == 4, "row groups per work-group");
+  constexpr int NP = 4 / G;  // parts of each group's columns
+  const int Rb = blockIdx.x * 8 * G;
   if (Rb >= R) return;  // the whole work-group
   const int tid = threadIdx.x, lane = tid & 63;
   const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
@@ -2574,7 +2582,7 @@ __global__ void __launch_bounds__(256) k_direct_band(const double *__restrict__ 
   // the windows in batches of QF entries per thread, the batch's loads
   // issued before its stores (one entry per trip: 8.6 against 8.2 us at
   // N = 4096; QF = 16 takes 174 VGPRs and was slower still)
-  constexpr int QF = 4;
+  constexpr int QF = 8;
   for (int e0 = tid; e0 < W; e0 += QF * 256) {
     double vx[QF], vp[QF], vh[QF];
 #pragma unroll
@@ -2597,29 +2605,44 @@ __global__ void __launch_bounds__(256) k_direct_band(const double *__restrict__ 
     }
   }
   __syncthreads();
-  const int r0 = Rb + 8 * wave;
-  if (r0 >= R) return;  // no barrier below
-  const int len = R - r0;
-  int T8 = (span + 511) / 512;
+  const int g = wave % G, c = wave / G;  // row group, column part
+  const int r0 = Rb + 8 * g;
+  const bool live = r0 < R;  // (every wave reaches the barrier below)
+  const int len = live ? R - r0 : 0;
+  int T8 = (span + 512 * NP - 1) / (512 * NP);
   T8 |= 1;
   const int T = 8 * T8;
-  const int t0 = lane * T, t1 = std::min(t0 + T, len);
+  const int t0 = (64 * c + lane) * T, t1 = std::min(t0 + T, len);
   double acc[8];
 #pragma unroll
   for (int k = 0; k < 8; k++) acc[k] = 0.0;
-  if (t0 < len)
-    direct_seg8(xw + dband_pad(8 * wave + t0), pw + dband_pad(16 * wave + t0), hw + dband_pad(t0),
-                t0, t1, len, acc);
+  if (t0 < len)  // false for every lane of a wave past the last row
+    direct_seg8(xw + dband_pad(8 * g + t0), pw + dband_pad(16 * g + t0), hw + dband_pad(t0), t0,
+                t1, len, acc);
+  // the 64 lanes' sums of the 8 rows, halving the rows at each of the
+  // first three exchanges (lanes with bit 5 set keep rows 4-7, bit 4 the
+  // upper two of those four, bit 3 the upper one): 10 exchanges instead of
+  // 48, and lane l ends with row l >> 3
+  const bool b5 = lane & 32, b4 = lane & 16, b3 = lane & 8;
+  double k4[4], k2[2];
 #pragma unroll
-  for (int k = 0; k < 8; k++) {
+  for (int j = 0; j < 4; j++)
+    k4[j] = (b5 ? acc[j + 4] : acc[j]) + __shfl_xor(b5 ? acc[j] : acc[j + 4], 32, 64);
 #pragma unroll
-    for (int d = 32; d >= 1; d >>= 1) acc[k] += __shfl_xor(acc[k], d, 64);
+  for (int j = 0; j < 2; j++)
+    k2[j] = (b4 ? k4[j + 2] : k4[j]) + __shfl_xor(b4 ? k4[j] : k4[j + 2], 16, 64);
+  double v = (b3 ? k2[1] : k2[0]) + __shfl_xor(b3 ? k2[0] : k2[1], 8, 64);
+#pragma unroll
+  for (int d = 4; d >= 1; d >>= 1) v += __shfl_xor(v, d, 64);
+  if constexpr (NP == 2) {
+    double *red = sm + 3 * WP;  // [group][row]: the second part's sums
+    if (c == 1 && (lane & 7) == 0) red[8 * g + (lane >> 3)] = v;
+    __syncthreads();
+    if (c == 0 && (lane & 7) == 0) v += red[8 * g + (lane >> 3)];
   }
-  double v = acc[0];
-#pragma unroll
-  for (int k = 1; k < 8; k++) v = lane == k ? acc[k] : v;
-  const int r = r0 + lane;
-  if (lane < 8 && r < R) {
+  if (c != 0 || !live) return;
+  const int r = r0 + (lane >> 3);
+  if ((lane & 7) == 0 && r < R) {
     const int i = 2 * r + sg;
     if (DIR == 0) {
       v *= k2Pi;
@@ -3006,13 +3029,20 @@ static void launch_direct_one(Prof *prof, int N, size_t dir, const double *a, co
                               const double *dn, const double *d_in, double *d_out,
                               const LineMap &lm, hipStream_t st, bool band) {
   if (N > kDirectBandMinN && N <= kDirectBandMaxN && band) {
-    const dim3 grid((unsigned)(((N + 1) / 2 + 31) / 32), 2);
-    if (dir == L2C)
-      launch_k(prof, k_direct_band<0>, grid, 256, dband_shared_bytes(N), st, d_in, d_out, lm, N,
-               a, an, dn);
-    else
-      launch_k(prof, k_direct_band<1>, grid, 256, dband_shared_bytes(N), st, d_in, d_out, lm, N,
-               a, an, dn);
+    const int G = dband_groups(N), rows = 8 * G;
+    const dim3 grid((unsigned)(((N + 1) / 2 + rows - 1) / rows), 2);
+    const size_t shm = dband_shared_bytes(N);
+    if (G == 2) {
+      if (dir == L2C)
+        launch_k(prof, k_direct_band<0, 2>, grid, 256, shm, st, d_in, d_out, lm, N, a, an, dn);
+      else
+        launch_k(prof, k_direct_band<1, 2>, grid, 256, shm, st, d_in, d_out, lm, N, a, an, dn);
+    } else {
+      if (dir == L2C)
+        launch_k(prof, k_direct_band<0, 4>, grid, 256, shm, st, d_in, d_out, lm, N, a, an, dn);
+      else
+        launch_k(prof, k_direct_band<1, 4>, grid, 256, shm, st, d_in, d_out, lm, N, a, an, dn);
+    }
     return;
   }
   launch_k(prof, k_direct_rows, dim3((unsigned)((N + 3) / 4)), 256, 0, st, d_in, d_out, lm, N,
