@@ -59,7 +59,11 @@ fmm_plan *create_fmm(const size_t N, const size_t maxs, const size_t M,
 
 /* Replaces execute, src/C/leg2cheb.c:1299-1500 (decl. leg2cheb.h:104-105).
  * Elements are input_array[i*stride], output_array[i*stride], i < N.
- * Returns the reference's algorithmic flop count, (size_t)-1 on error. */
+ * Returns the reference's algorithmic flop count, (size_t)-1 on error.
+ * Plans of N <= 8192 compute the transform as the direct O(N^2) sums in one
+ * launch (faster there than the FMM's three; environment L2C_DIRECT_MAXN at
+ * plan creation moves the bound, 0 keeps the FMM); l2c_execute_device and
+ * the plan's graph replays do the same, line calls keep the FMM. */
 size_t execute(const double *input_array, double *output_array,
                fmm_plan *fmmplan, size_t direction, const size_t stride);
 
