@@ -132,3 +132,22 @@ def test_lines_keep_fmm(amd, monkeypatch):
     y = dout.download().reshape(B, N)
     for b in range(B):
         assert relerr(y[b], p.execute(u[b * N:(b + 1) * N], 0)) <= tol(N)
+
+
+@pytest.mark.parametrize("lagrange,precompute", [(1, 0), (0, 1), (1, 1)])
+@pytest.mark.parametrize("maxs", [32, 64])
+def test_small_n_plan_variants(amd, orc, monkeypatch, lagrange, precompute, maxs):
+    """The nodal (lagrange=1) and precompute plans and another leaf size take
+    the same direct route: the transform, not the plan's FMM variant, is
+    what they compute."""
+    N = 4096
+    announce(f"small-N variants lagrange={lagrange} precompute={precompute} maxs={maxs}")
+    u = amd.coefficients(N, seed=15)
+    monkeypatch.delenv("L2C_DIRECT_MAXN", raising=False)
+    for d in (0, 1):
+        p = amd.FMMPlan(N, maxs, 18, d, lagrange, precompute)
+        assert route(p, lambda: p.execute(u, d)) == {"k_direct"}
+        g = p.execute(u, d)
+        o = orc.FMMPlan(N, maxs, 18, d, lagrange, precompute).execute(u, d)
+        ref = orc.leg2cheb_dd(u) if d == 0 else orc.cheb2leg_dd(u)
+        assert relerr(g, o) <= tol(N) and relerr(g, ref) <= tol(N)
