@@ -2487,7 +2487,8 @@ __global__ void __launch_bounds__(256) k_direct_rows(const double *__restrict__ 
 // with H = P = Lambda, X = u (L2C), or H = dn (H(0) = 0: the diagonal is
 // added after the sum), P = an, X(m) = u_n n (C2L).  A work-group of four
 // waves owns 8 G rows of one parity (G groups of 8 rows, 4 / G waves each)
-// and stages X, P and H for every column its rows touch in LDS; lane j of
+// and stages X, P and H for the columns its rows touch in LDS, in chunks of
+// kDbChunk columns (one chunk up to N = 8192); per chunk, lane j of
 // the wave for part c of group g takes the columns t = m - r0 in
 // [(64c + j) T, (64c + j + 1) T) of the group's 8 rows, T a multiple of 8
 // with T/8 odd; the 64 lanes' sums are added across the wave, then the
@@ -2496,19 +2497,23 @@ __global__ void __launch_bounds__(256) k_direct_rows(const double *__restrict__ 
 // banks.  G = 2 up to N = 6144 (kernel trace at 4096: 7.3 against 8.0 us
 // with G = 4), G = 4 above (8192: 11.8 against 13.0 us: the windows, loaded
 // once per work-group, then cost more than the shorter column runs save).
-constexpr int kDirectBandMinN = 3072, kDirectBandMaxN = 8192;
-__host__ __device__ inline int dband_win(int R) { return R + 128; }
+constexpr int kDirectBandMinN = 3072;
+// columns per LDS window load: a work-group whose rows touch more columns
+// runs them in chunks of this many, the windows reloaded for each
+constexpr int kDbChunk = 4096;
+__host__ __device__ inline int dband_win(int cols) { return cols + 128; }
 __host__ __device__ inline int dband_pad(int e) { return e + (e >> 3); }
 constexpr int kDbSplitMaxN = 6144;  // G = 2 up to this N, else 4
 inline int dband_groups(int N) { return N <= kDbSplitMaxN ? 2 : 4; }
 size_t dband_shared_bytes(int N) {
-  const int w = dband_pad(dband_win((N + 1) / 2)) + 1;
+  const int w = dband_pad(dband_win(std::min((N + 1) / 2, kDbChunk))) + 1;
   return 8 * (size_t)(3 * w + 16);  // windows, the second parts' row sums (G = 2)
 }
 
 // Columns [t0, t1) of the lane's 8 rows, t0 a multiple of 8; xq, pq, hq: the
-// padded blocks of X(r0 + t0), P(2 r0 + sg + t0), H(t0).  Columns t >= len
-// read X as zero.  Rings as in band_seg8: slot t mod 8 holds H(t - 8 .. t)
+// padded blocks of X(r0 + t0), P(2 r0 + sg + t0), H(t0), the block before
+// hq holding H(t0 - 8 .. t0 - 1) (zero below 0).  Columns t >= len read X as
+// zero.  Rings as in band_seg8: slot t mod 8 holds H(t - 8 .. t)
 // and P(t .. t + 7); row rho of column t takes H(t - rho) P(t + rho) X(t).
 __device__ __forceinline__ void direct_seg8(const double *__restrict__ xq,
                                             const double *__restrict__ pq,
@@ -2519,7 +2524,7 @@ __device__ __forceinline__ void direct_seg8(const double *__restrict__ xq,
   double hr[8], pr[8];
 #pragma unroll
   for (int k = 0; k < 8; k++) {
-    hr[k] = t0 >= 8 ? hq[k - 9] : 0.0;
+    hr[k] = hq[k - 9];
     pr[k] = pq[k];
   }
   int tt = t0;
@@ -2571,7 +2576,8 @@ __global__ void __launch_bounds__(256) k_direct_band(const double *__restrict__ 
   const int tid = threadIdx.x, lane = tid & 63;
   const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
   const int span = R - Rb;  // columns Rb .. R - 1
-  const int W = dband_win(span), WP = dband_pad(W) + 1;
+  const int nch = (span + kDbChunk - 1) / kDbChunk;  // column chunks
+  const int W = dband_win(std::min(span, kDbChunk)), WP = dband_pad(W) + 1;
   double *xw = sm, *pw = sm + WP, *hw = sm + 2 * WP;
   const size_t off = line_off(lm, 0);
   const double *u = in + off;
@@ -2579,46 +2585,54 @@ __global__ void __launch_bounds__(256) k_direct_band(const double *__restrict__ 
   const double *Ht = DIR == 0 ? a : dn;
   const double *Pt = DIR == 0 ? a : an;
   const int nH = DIR == 0 ? N : (N + 1) / 2;  // entries of the H table
-  // the windows in batches of QF entries per thread, the batch's loads
-  // issued before its stores (one entry per trip: 8.6 against 8.2 us at
-  // N = 4096; QF = 16 takes 174 VGPRs and was slower still)
-  constexpr int QF = 8;
-  for (int e0 = tid; e0 < W; e0 += QF * 256) {
-    double vx[QF], vp[QF], vh[QF];
-#pragma unroll
-    for (int k = 0; k < QF; k++) {
-      const int e = e0 + k * 256;
-      const int m = Rb + e, qi = 2 * Rb + sg + e;
-      vx[k] = m < R ? u[(size_t)(2 * m + sg) * es] : 0.0;
-      vp[k] = qi < N ? Pt[qi] : 0.0;
-      vh[k] = (e < span && e < nH && !(DIR == 1 && e == 0)) ? Ht[e] : 0.0;
-    }
-#pragma unroll
-    for (int k = 0; k < QF; k++) {
-      const int e = e0 + k * 256;
-      if (e < W) {
-        const int pe = dband_pad(e);
-        xw[pe] = DIR == 1 ? vx[k] * (double)(2 * (Rb + e) + sg) : vx[k];
-        pw[pe] = vp[k];
-        hw[pe] = vh[k];
-      }
-    }
-  }
-  __syncthreads();
   const int g = wave % G, c = wave / G;  // row group, column part
   const int r0 = Rb + 8 * g;
-  const bool live = r0 < R;  // (every wave reaches the barrier below)
+  const bool live = r0 < R;  // (every wave reaches the barriers below)
   const int len = live ? R - r0 : 0;
-  int T8 = (span + 512 * NP - 1) / (512 * NP);
-  T8 |= 1;
-  const int T = 8 * T8;
-  const int t0 = (64 * c + lane) * T, t1 = std::min(t0 + T, len);
   double acc[8];
 #pragma unroll
   for (int k = 0; k < 8; k++) acc[k] = 0.0;
-  if (t0 < len)  // false for every lane of a wave past the last row
-    direct_seg8(xw + dband_pad(8 * g + t0), pw + dband_pad(16 * g + t0), hw + dband_pad(t0), t0,
-                t1, len, acc);
+  for (int ch = 0; ch < nch; ch++) {
+    // columns t = m - r0 in [tb, tb + kDbChunk): windows X(Rb + tb + e),
+    // P(2 Rb + sg + tb + e), H(tb - 8 + e), e < W, in batches of QF
+    // entries per thread, the batch's loads issued before its stores (one
+    // entry per trip: 8.6 against 8.2 us at N = 4096; QF = 16 takes 174
+    // VGPRs and was slower still)
+    const int tb = ch * kDbChunk;
+    constexpr int QF = 8;
+    for (int e0 = tid; e0 < W; e0 += QF * 256) {
+      double vx[QF], vp[QF], vh[QF];
+#pragma unroll
+      for (int k = 0; k < QF; k++) {
+        const int e = e0 + k * 256;
+        const int m = Rb + tb + e, qi = 2 * Rb + sg + tb + e, hk = tb - 8 + e;
+        vx[k] = m < R ? u[(size_t)(2 * m + sg) * es] : 0.0;
+        vp[k] = qi < N ? Pt[qi] : 0.0;
+        vh[k] = (hk >= 0 && hk < span && hk < nH && !(DIR == 1 && hk == 0)) ? Ht[hk] : 0.0;
+      }
+#pragma unroll
+      for (int k = 0; k < QF; k++) {
+        const int e = e0 + k * 256;
+        if (e < W) {
+          const int pe = dband_pad(e);
+          xw[pe] = DIR == 1 ? vx[k] * (double)(2 * (Rb + tb + e) + sg) : vx[k];
+          pw[pe] = vp[k];
+          hw[pe] = vh[k];
+        }
+      }
+    }
+    __syncthreads();
+    const int cw = std::min(kDbChunk, span - tb);
+    int T8 = (cw + 512 * NP - 1) / (512 * NP);
+    T8 |= 1;
+    const int T = 8 * T8;
+    const int t0 = tb + (64 * c + lane) * T;
+    const int t1 = std::min(std::min(t0 + T, tb + kDbChunk), len);
+    if (t0 < t1)  // false for every lane of a wave past the last row
+      direct_seg8(xw + dband_pad(8 * g + t0 - tb), pw + dband_pad(16 * g + t0 - tb),
+                  hw + dband_pad(t0 - tb + 8), t0, t1, len, acc);
+    if (ch + 1 < nch) __syncthreads();  // the next chunk's loads overwrite the windows
+  }
   // the 64 lanes' sums of the 8 rows, halving the rows at each of the
   // first three exchanges (lanes with bit 5 set keep rows 4-7, bit 4 the
   // upper two of those four, bit 3 the upper one): 10 exchanges instead of
@@ -3019,16 +3033,16 @@ size_t gemm_shared_bytes() {
 }
 
 // One transform by direct sums from the Lambda tables a, an, dn (those of a
-// direct plan of size N, or of an FMM plan): k_direct_band for N in
-// (kDirectBandMinN, kDirectBandMaxN] unless band is false (the plan's
-// direct_band), else k_direct_rows.  Kernel trace, L2C, k_direct_band /
+// direct plan of size N, or of an FMM plan): k_direct_band for N above
+// kDirectBandMinN unless band is false (the plan's direct_band), else
+// k_direct_rows.  Kernel trace, L2C, k_direct_band /
 // k_direct_rows: N = 1024 5.8 / 3.8 us, 4096 8.2 / 9.7 us, 8192 12.2 /
 // 27.8 us (below about 3000 the band kernel's window loads and cross-lane
 // sums cost more than the row kernel's loads save).
 static void launch_direct_one(Prof *prof, int N, size_t dir, const double *a, const double *an,
                               const double *dn, const double *d_in, double *d_out,
                               const LineMap &lm, hipStream_t st, bool band) {
-  if (N > kDirectBandMinN && N <= kDirectBandMaxN && band) {
+  if (N > kDirectBandMinN && band) {
     const int G = dband_groups(N), rows = 8 * G;
     const dim3 grid((unsigned)(((N + 1) / 2 + rows - 1) / rows), 2);
     const size_t shm = dband_shared_bytes(N);

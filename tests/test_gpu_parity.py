@@ -295,15 +295,16 @@ def test_upward_level_forms(amd, orc, direction, nlines):
         assert e <= tol(N)
 
 
-@pytest.mark.parametrize("N", [1023, 1024, 3001, 3073, 5000, 8192])
+@pytest.mark.parametrize("N", [1023, 1024, 3001, 3073, 5000, 8192, 12345, 32768])
 @pytest.mark.parametrize("direction", [0, 1])
 @pytest.mark.parametrize("stride", [1, 3])
 def test_direct_api_rows(amd, orc, N, direction, stride):
     """The reference's direct() (src/C/leg2cheb.c:152-234) on a direct plan:
     below 1024 one thread per row (k_direct), from 1024 on one wave per row
-    (k_direct_rows), from 3073 to 8192 the band kernel's register rings
-    (k_direct_band); strided arrays, both directions, against the oracle's
-    direct sums."""
+    (k_direct_rows), from 3073 on the band kernel's register rings
+    (k_direct_band; past 8192 its rows' columns in several LDS window
+    chunks); strided arrays, both directions, against the oracle's direct
+    sums."""
     announce(f"direct() N={N} dir={direction} stride={stride}")
     L = amd.lib()
     u = amd.coefficients(N, seed=41 + N + direction, decay=1.0 * direction)
