@@ -255,7 +255,7 @@ struct direct_plan_s {
   // lines per grid row of k_direct (grid.y limit 65535; L2C_DIRECT_LINES_Y
   // sets a smaller value, which the tests use to reach grid.z > 1 cheaply)
   unsigned lines_y = 65535;
-  // single transforms up to N = 8192 on k_direct_band, else k_direct_rows
+  // single transforms of N > 3072 on k_direct_band, else k_direct_rows
   // (L2C_DIRECT_BAND=0: always k_direct_rows)
   bool direct_band = true;
   double *d_in = nullptr, *d_out = nullptr;  // staging (device)
@@ -293,7 +293,7 @@ struct fmm_plan_s {
   // instead of the three FMM launches, whose dependent chains set a floor of
   // about 22 us whatever the size; 0: always the FMM; L2C_DIRECT_MAXN
   int direct_max_n = 8192;
-  // single direct transforms up to N = 8192 on k_direct_band (the band's
+  // single direct transforms of N > 3072 on k_direct_band (the band's
   // register rings over LDS windows) rather than k_direct_rows (three loads
   // per term); L2C_DIRECT_BAND=0 selects k_direct_rows
   bool direct_band = true;
