@@ -63,7 +63,8 @@ fmm_plan *create_fmm(const size_t N, const size_t maxs, const size_t M,
  * Plans of N <= 8192 compute the transform as the direct O(N^2) sums in one
  * launch (faster there than the FMM's three; environment L2C_DIRECT_MAXN at
  * plan creation moves the bound, 0 keeps the FMM); l2c_execute_device and
- * the plan's graph replays do the same, line calls keep the FMM. */
+ * the plan's graph replays do the same, and so do line calls of at most
+ * 16,384 elements (environment L2C_DIRECT_LINES). */
 size_t execute(const double *input_array, double *output_array,
                fmm_plan *fmmplan, size_t direction, const size_t stride);
 

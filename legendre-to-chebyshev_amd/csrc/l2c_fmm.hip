@@ -2444,7 +2444,8 @@ __global__ void __launch_bounds__(256) k_direct(const double *__restrict__ in,
 // (one thread per row leaves a few dozen waves, each a serial loop of N/2
 // terms): the wave's lanes take every 64th term of the row, in the same
 // term order per lane, and add their partial sums across the wave.  Grid
-// ((N + 3) / 4): four rows per work-group.
+// ((N + 3) / 4, lines): four rows per work-group, line blockIdx.y of lm
+// (unpacked maps).
 __global__ void __launch_bounds__(256) k_direct_rows(const double *__restrict__ in,
                                                      double *__restrict__ out, LineMap lm, int N,
                                                      int dir, const double *__restrict__ a,
@@ -2452,7 +2453,7 @@ __global__ void __launch_bounds__(256) k_direct_rows(const double *__restrict__ 
                                                      const double *__restrict__ dn) {
   const int i = blockIdx.x * 4 + (threadIdx.x >> 6), lane = threadIdx.x & 63;
   if (i >= N) return;  // whole waves: the row index is wave-uniform
-  const size_t off = line_off(lm, 0);
+  const size_t off = line_off(lm, blockIdx.y);
   const double *u = in + off;
   const size_t es = lm.es;
   // (four independent sums per lane, so that a trip's loads are in flight
@@ -2558,8 +2559,9 @@ __device__ __forceinline__ void direct_seg8(const double *__restrict__ xq,
 #undef DFETCH
 }
 
-// Grid ((R0 + 8 G - 1) / (8 G), 2) with R0 = (N + 1) / 2, 256 threads,
-// dband_shared_bytes(N) of LDS.
+// Grid ((R0 + 8 G - 1) / (8 G), 2, lines) with R0 = (N + 1) / 2, 256
+// threads, dband_shared_bytes(N) of LDS; line blockIdx.z of lm (unpacked
+// maps).
 template <int DIR, int G>
 __global__ void __launch_bounds__(256) k_direct_band(const double *__restrict__ in,
                                                      double *__restrict__ out, LineMap lm, int N,
@@ -2579,7 +2581,7 @@ __global__ void __launch_bounds__(256) k_direct_band(const double *__restrict__ 
   const int nch = (span + kDbChunk - 1) / kDbChunk;  // column chunks
   const int W = dband_win(std::min(span, kDbChunk)), WP = dband_pad(W) + 1;
   double *xw = sm, *pw = sm + WP, *hw = sm + 2 * WP;
-  const size_t off = line_off(lm, 0);
+  const size_t off = line_off(lm, blockIdx.z);
   const double *u = in + off;
   const size_t es = lm.es;
   const double *Ht = DIR == 0 ? a : dn;
@@ -3041,10 +3043,10 @@ size_t gemm_shared_bytes() {
 // sums cost more than the row kernel's loads save).
 static void launch_direct_one(Prof *prof, int N, size_t dir, const double *a, const double *an,
                               const double *dn, const double *d_in, double *d_out,
-                              const LineMap &lm, hipStream_t st, bool band) {
+                              const LineMap &lm, hipStream_t st, bool band, unsigned nlines = 1) {
   if (N > kDirectBandMinN && band) {
     const int G = dband_groups(N), rows = 8 * G;
-    const dim3 grid((unsigned)(((N + 1) / 2 + rows - 1) / rows), 2);
+    const dim3 grid((unsigned)(((N + 1) / 2 + rows - 1) / rows), 2, nlines);
     const size_t shm = dband_shared_bytes(N);
     if (G == 2) {
       if (dir == L2C)
@@ -3059,8 +3061,8 @@ static void launch_direct_one(Prof *prof, int N, size_t dir, const double *a, co
     }
     return;
   }
-  launch_k(prof, k_direct_rows, dim3((unsigned)((N + 3) / 4)), 256, 0, st, d_in, d_out, lm, N,
-           (int)dir, a, an, dn);
+  launch_k(prof, k_direct_rows, dim3((unsigned)((N + 3) / 4), nlines), 256, 0, st, d_in, d_out,
+           lm, N, (int)dir, a, an, dn);
 }
 
 // k_direct over nlines lines of map lm (one launch for any number of lines);
@@ -3454,6 +3456,16 @@ int reserve_lines(fmm_plan_s *p, size_t nlines) {
   return ensure_ws(p, std::min<size_t>(nlines, 65535), g, &w);
 }
 
+// Whether a line call of nlines lines of FMM plan p runs as direct sums in
+// one launch: the plan's small-N route (direct_max_n) while the call holds at
+// most direct_line_elems elements, where the FMM's fixed cost of about 22 us
+// per call is still the larger.  Per call, FMM / direct (us): N = 2048 with
+// 8 lines 21.7 / 16.6 (16 lines 21.8 / 29.0), 4096 with 4 lines 23.1 / 14.3
+// (8 lines 23.2 / 23.9), 8192 with 2 lines 24.5 / 19.1 (3 lines 24.6 / 25.9).
+static bool direct_lines(const fmm_plan_s *p, size_t nlines) {
+  return p->N <= (size_t)p->direct_max_n && nlines * p->N <= (size_t)p->direct_line_elems;
+}
+
 int run_lines(fmm_plan_s *p, const double *d_in, double *d_out, size_t dir, size_t nlines,
               size_t n0, size_t ls0, size_t ls1, size_t es, hipStream_t st) {
   prof_step(p->prof);
@@ -3466,6 +3478,14 @@ int run_lines(fmm_plan_s *p, const double *d_in, double *d_out, size_t dir, size
     const int rc = launch_direct_lines(dp, d_in, d_out, dir, lm, nlines, st);
     prof_end(dp->prof, st);
     return rc;
+  }
+  if (direct_lines(p, nlines)) {
+    prof_begin(p->prof, K_DIRECT, st);
+    launch_direct_one(p->prof, (int)p->N, dir, p->t.a, p->t.an, p->t.dn, d_in, d_out, lm, st,
+                      p->direct_band, (unsigned)nlines);
+    prof_end(p->prof, st);
+    L2C_HIP(hipGetLastError(), -1);
+    return 0;
   }
   // lines go in grid.y: at most 65535 per launch, each chunk reusing the
   // front of the plan's line workspace
@@ -3521,6 +3541,12 @@ static int run_lines_scratch(fmm_plan_s *p, double *scr, const double *d_in, dou
                              size_t es, hipStream_t st) {
   LineMap lm{n0 ? n0 : 1, ls0, ls1, es};
   if (p->L == 0) return launch_direct_lines(p->dplan, d_in, d_out, dir, lm, nlines, st);
+  if (direct_lines(p, nlines)) {  // as l2c_execute_lines_device: no workspace
+    launch_direct_one(p->prof, (int)p->N, dir, p->t.a, p->t.an, p->t.dn, d_in, d_out, lm, st,
+                      p->direct_band, (unsigned)nlines);
+    L2C_HIP(hipGetLastError(), -1);
+    return 0;
+  }
   const size_t wsd = std::max(ws_doubles(p->geo), ws_doubles(p->geo1));
   const size_t per = std::min<size_t>(65535, kScratchDoubles / wsd);
   if (per == 0) {

@@ -293,6 +293,9 @@ struct fmm_plan_s {
   // instead of the three FMM launches, whose dependent chains set a floor of
   // about 22 us whatever the size; 0: always the FMM; L2C_DIRECT_MAXN
   int direct_max_n = 8192;
+  // line calls of such plans of at most this many elements (lines x N) take
+  // the same direct sums, all lines in one launch; L2C_DIRECT_LINES
+  int direct_line_elems = 16384;
   // single direct transforms of N > 3072 on k_direct_band (the band's
   // register rings over LDS windows) rather than k_direct_rows (three loads
   // per term); L2C_DIRECT_BAND=0 selects k_direct_rows

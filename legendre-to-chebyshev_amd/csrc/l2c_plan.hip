@@ -605,6 +605,7 @@ fmm_plan_s *create_fmm_impl(size_t N, size_t maxs, size_t M, size_t direction, s
   if (const char *e = getenv("L2C_ROLES_SHIFT")) p->roles_shift = atoi(e);
   if (const char *e = getenv("L2C_DIRECT_MAXN")) p->direct_max_n = std::max(0, atoi(e));
   if (const char *e = getenv("L2C_DIRECT_BAND")) p->direct_band = atoi(e) != 0;
+  if (const char *e = getenv("L2C_DIRECT_LINES")) p->direct_line_elems = std::max(0, atoi(e));
   (void)hipGetDevice(&p->device);
   if (hipStreamCreateWithFlags(&p->stream, hipStreamNonBlocking) != hipSuccess) {
     set_error("hipStreamCreate failed");

@@ -2,7 +2,8 @@
 plan field direct_max_n, L2C_DIRECT_MAXN) execute() and l2c_execute_device
 run the direct sums of src/C/leg2cheb.c:152-234 in one launch
 (k_direct_rows, or k_direct_band from N = 3073) instead of the three FMM
-launches.  Checked against the oracle's FMM (the reference's execute) and the
+launches, and so do l2c_execute_lines_device calls of at most 16,384
+elements (L2C_DIRECT_LINES), every line in one launch.  Checked against the oracle's FMM (the reference's execute) and the
 double-double direct sums, tolerance 1e-14 log2 N, and against the plan's own
 FMM route."""
 import math
@@ -118,20 +119,47 @@ def test_above_threshold_keeps_fmm(amd, orc, monkeypatch, N):
     assert relerr(p.execute(u, 0), orc.FMMPlan(N, 64, 18, 0).execute(u, 0)) <= tol(N)
 
 
-def test_lines_keep_fmm(amd, monkeypatch):
-    """Line calls (nlines > 1) of a small plan stay on the FMM."""
-    announce("small-N lines stay FMM")
-    N, B = 4096, 4
-    p = make_plan(amd, monkeypatch, N, 64, 0)
+@pytest.mark.parametrize("N,B,route_slot", [(4096, 4, "k_direct"), (4096, 5, "k_up"),
+                                             (8192, 2, "k_direct"), (2048, 8, "k_direct"),
+                                             (8192, 3, "k_up")])
+def test_small_n_lines_route(amd, orc, monkeypatch, N, B, route_slot):
+    """Line calls of a small plan holding at most 16,384 elements take the
+    direct sums in one launch (all lines in grid.z), larger ones the FMM;
+    either way every line matches the oracle and the single-line call."""
+    announce(f"small-N lines N={N} B={B}")
+    p = make_plan(amd, monkeypatch, N, 64, 2)
     u = np.concatenate([amd.coefficients(N, seed=60 + b) for b in range(B)])
     din, dout = amd.DeviceBuffer(B * N), amd.DeviceBuffer(B * N)
     din.upload(u)
-    slots = route(p, lambda: (p.execute_lines_device(din.ptr, dout.ptr, 0, B, B, N, 0, 1),
+    for d in (0, 1):
+        slots = route(p, lambda: (p.execute_lines_device(din.ptr, dout.ptr, d, B, B, N, 0, 1),
+                                  p.synchronize()))
+        assert route_slot in slots
+        assert ("k_direct" in slots) == (route_slot == "k_direct")
+        y = dout.download().reshape(B, N)
+        op = orc.FMMPlan(N, 64, 18, d)
+        for b in range(B):
+            ub = u[b * N:(b + 1) * N]
+            assert relerr(y[b], op.execute(ub, d)) <= tol(N)
+            assert relerr(y[b], p.execute(ub, d)) <= tol(N)
+
+
+def test_small_n_lines_strided(amd, orc, monkeypatch):
+    """Two interleaved lines (element stride 2, line offsets 0 and 1) of
+    N = 4096 through the direct route."""
+    announce("small-N interleaved lines")
+    N = 4096
+    p = make_plan(amd, monkeypatch, N, 64, 0)
+    x = np.stack([amd.coefficients(N, seed=80), amd.coefficients(N, seed=81)], axis=1).ravel()
+    din, dout = amd.DeviceBuffer(2 * N), amd.DeviceBuffer(2 * N)
+    din.upload(x)
+    slots = route(p, lambda: (p.execute_lines_device(din.ptr, dout.ptr, 0, 2, 2, 1, 0, 2),
                               p.synchronize()))
-    assert "k_up" in slots and "k_direct" not in slots
-    y = dout.download().reshape(B, N)
-    for b in range(B):
-        assert relerr(y[b], p.execute(u[b * N:(b + 1) * N], 0)) <= tol(N)
+    assert slots == {"k_direct"}
+    y = dout.download().reshape(N, 2)
+    op = orc.FMMPlan(N, 64, 18, 0)
+    for b in range(2):
+        assert relerr(y[:, b], op.execute(np.ascontiguousarray(x.reshape(N, 2)[:, b]), 0)) <= tol(N)
 
 
 @pytest.mark.parametrize("lagrange,precompute", [(1, 0), (0, 1), (1, 1)])

@@ -2,8 +2,8 @@
 its FMM workspace is a fixed device array) against l2c_execute_lines_device
 through the C ABI: identical outputs on two calls in a row, the second with
 fewer lines, for FMM plans (several launches when the lines outgrow the
-array) and a direct-path plan; a plan whose single line outgrows the array
-is refused before any launch."""
+array; few lines of a small plan as direct sums) and a direct-path plan; a
+plan whose single line outgrows the array is refused before any launch."""
 import math
 
 import numpy as np
@@ -44,6 +44,27 @@ def test_noalloc_matches_lines_entry(amd, orc, N, maxs, n1, n2):
             b = _lines(amd, L.l2c_execute_lines_device, plan, x, d, nl, N)
             assert np.array_equal(a, b)
             for line in (0, nl - 1):
+                ref = op.execute(np.ascontiguousarray(x[line * N:(line + 1) * N]), d)
+                assert relerr(a[line * N:(line + 1) * N], ref) <= tol(N)
+
+
+def test_noalloc_direct_route(amd, orc, monkeypatch):
+    """With the default small-N route a call of at most 16,384 elements of
+    an FMM plan runs as direct sums in both entry points (no workspace);
+    4 lines of 4096, then 2, identical outputs."""
+    monkeypatch.delenv("L2C_DIRECT_MAXN", raising=False)
+    N = 4096
+    L = amd.lib()
+    plan = amd.FMMPlan(N, 64, 18, amd.BOTH)
+    op = orc.FMMPlan(N, 64, 18, orc.BOTH)
+    for d in (0, 1):
+        for nl in (4, 2):
+            announce(f"noalloc direct route N={N} lines={nl} dir={d}")
+            x = np.random.default_rng(N + nl + d).uniform(-1, 1, nl * N)
+            a = _lines(amd, L.l2c_execute_lines_device_noalloc, plan, x, d, nl, N)
+            b = _lines(amd, L.l2c_execute_lines_device, plan, x, d, nl, N)
+            assert np.array_equal(a, b)
+            for line in range(nl):
                 ref = op.execute(np.ascontiguousarray(x[line * N:(line + 1) * N]), d)
                 assert relerr(a[line * N:(line + 1) * N], ref) <= tol(N)
 
