@@ -2481,8 +2481,9 @@ __global__ void __launch_bounds__(256) k_direct_rows(const double *__restrict__ 
 }
 
 // ---------------------------------------------------------------------------
-// Direct sums of a single transform of small N with the band's register
-// rings (src/C/leg2cheb.c:152-234).  Per parity sg the rows r and columns m
+// Direct sums of single transforms (and of line calls of a few lines, one
+// line per grid.z) of N above 3072 with the band's register rings
+// (src/C/leg2cheb.c:152-234).  Per parity sg the rows r and columns m
 // (element 2m + sg) run over [0, R), R = (N - sg + 1) / 2, and
 //   y_r = sum_{m >= r} H(m - r) P(m + r + sg) X(m)
 // with H = P = Lambda, X = u (L2C), or H = dn (H(0) = 0: the diagonal is
