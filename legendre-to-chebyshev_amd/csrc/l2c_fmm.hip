@@ -2496,21 +2496,23 @@ __global__ void __launch_bounds__(256) k_direct_rows(const double *__restrict__ 
 // with T/8 odd; the 64 lanes' sums are added across the wave, then the
 // parts through LDS.  Every window stores index e at e + e/8, so the 64
 // lanes' addresses (64 segments of T/8 padded blocks apart) fall in distinct
-// banks.  G = 2 up to N = 6144 (kernel trace at 4096: 7.3 against 8.0 us
-// with G = 4), G = 4 above (8192: 11.8 against 13.0 us: the windows, loaded
-// once per work-group, then cost more than the shorter column runs save).
+// banks.  G = 1 (8 rows, four parts) while two work-groups' windows fit a
+// CU, else G = 4 (32 rows, one part).  Kernel trace (us), G = 1 / 2 / 4: N =
+// 3500 6.4 / 6.4 / 7.0, 4096 7.1 / 7.4 / 7.7, 6000 15.6 / 10.2 / 9.8, 8192
+// 20.9 / 12.7 / 11.0 (from 6000 on one work-group per CU: G = 1 then takes
+// two rounds, and more rows per work-group load the windows fewer times).
 constexpr int kDirectBandMinN = 3072;
 // columns per LDS window load: a work-group whose rows touch more columns
 // runs them in chunks of this many, the windows reloaded for each
 constexpr int kDbChunk = 4096;
 __host__ __device__ inline int dband_win(int cols) { return cols + 128; }
 __host__ __device__ inline int dband_pad(int e) { return e + (e >> 3); }
-constexpr int kDbSplitMaxN = 6144;  // G = 2 up to this N, else 4
-inline int dband_groups(int N) { return N <= kDbSplitMaxN ? 2 : 4; }
 size_t dband_shared_bytes(int N) {
   const int w = dband_pad(dband_win(std::min((N + 1) / 2, kDbChunk))) + 1;
-  return 8 * (size_t)(3 * w + 16);  // windows, the second parts' row sums (G = 2)
+  return 8 * (size_t)(3 * w + 24);  // windows, the other parts' row sums (G = 1)
 }
+// G = 1 while two work-groups' windows fit one CU's 160 KB of LDS, else 4
+inline int dband_groups(int N) { return dband_shared_bytes(N) <= 80 * 1024 ? 1 : 4; }
 
 // Columns [t0, t1) of the lane's 8 rows, t0 a multiple of 8; xq, pq, hq: the
 // padded blocks of X(r0 + t0), P(2 r0 + sg + t0), H(t0), the block before
@@ -2572,7 +2574,7 @@ __global__ void __launch_bounds__(256) k_direct_band(const double *__restrict__ 
   extern __shared__ double sm[];
   const int sg = blockIdx.y;
   const int R = (N - sg + 1) / 2;
-  static_assert(G == 2 || G == 4, "row groups per work-group");
+  static_assert(G == 1 || G == 4, "row groups per work-group");
   constexpr int NP = 4 / G;  // parts of each group's columns
   const int Rb = blockIdx.x * 8 * G;
   if (Rb >= R) return;  // the whole work-group
@@ -2651,11 +2653,14 @@ __global__ void __launch_bounds__(256) k_direct_band(const double *__restrict__ 
   double v = (b3 ? k2[1] : k2[0]) + __shfl_xor(b3 ? k2[0] : k2[1], 8, 64);
 #pragma unroll
   for (int d = 4; d >= 1; d >>= 1) v += __shfl_xor(v, d, 64);
-  if constexpr (NP == 2) {
-    double *red = sm + 3 * WP;  // [group][row]: the second part's sums
-    if (c == 1 && (lane & 7) == 0) red[8 * g + (lane >> 3)] = v;
+  if constexpr (NP > 1) {
+    double *red = sm + 3 * WP;  // [part - 1][group][row]: the other parts' sums
+    if (c > 0 && (lane & 7) == 0) red[(c - 1) * 8 * G + 8 * g + (lane >> 3)] = v;
     __syncthreads();
-    if (c == 0 && (lane & 7) == 0) v += red[8 * g + (lane >> 3)];
+    if (c == 0 && (lane & 7) == 0) {
+#pragma unroll
+      for (int q = 1; q < NP; q++) v += red[(q - 1) * 8 * G + 8 * g + (lane >> 3)];
+    }
   }
   if (c != 0 || !live) return;
   const int r = r0 + (lane >> 3);
@@ -3049,11 +3054,11 @@ static void launch_direct_one(Prof *prof, int N, size_t dir, const double *a, co
     const int G = dband_groups(N), rows = 8 * G;
     const dim3 grid((unsigned)(((N + 1) / 2 + rows - 1) / rows), 2, nlines);
     const size_t shm = dband_shared_bytes(N);
-    if (G == 2) {
+    if (G == 1) {
       if (dir == L2C)
-        launch_k(prof, k_direct_band<0, 2>, grid, 256, shm, st, d_in, d_out, lm, N, a, an, dn);
+        launch_k(prof, k_direct_band<0, 1>, grid, 256, shm, st, d_in, d_out, lm, N, a, an, dn);
       else
-        launch_k(prof, k_direct_band<1, 2>, grid, 256, shm, st, d_in, d_out, lm, N, a, an, dn);
+        launch_k(prof, k_direct_band<1, 1>, grid, 256, shm, st, d_in, d_out, lm, N, a, an, dn);
     } else {
       if (dir == L2C)
         launch_k(prof, k_direct_band<0, 4>, grid, 256, shm, st, d_in, d_out, lm, N, a, an, dn);
