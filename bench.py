@@ -591,7 +591,9 @@ def main():
                     help="host-pointer execute() calls timed for the PCIe-inclusive rate; 0 = skip")
     ap.add_argument("--lines", type=int, default=8,
                     help="also time this many vectors of length N per step in one batched call; "
-                         "0 or 1 = skip")
+                         "0 or 1 = skip (this and --lines-more)")
+    ap.add_argument("--lines-more", type=int, default=32,
+                    help="and this many (reported as lines_per_step_more); 0 or 1 = skip")
     ap.add_argument("--cold-steps", type=int, default=40,
                     help="transforms timed after a 256 MiB device copy has evicted the "
                          "last-level cache (cold-cache kernel times); 0 = skip")
@@ -832,9 +834,7 @@ def main():
     # the same transform on B independent vectors per step, one batched
     # launch sequence (l2c_execute_lines_device): what the GPU sustains when
     # transforms of one size arrive together (reported beside the value)
-    lines = None
-    if args.lines > 1:
-        B = args.lines
+    def time_lines(B):
         ub = np.concatenate([amd.coefficients(N, seed=1 + b, decay=args.decay) for b in range(B)])
         bi, bo = amd.DeviceBuffer(B * N), amd.DeviceBuffer(B * N)
         bi.upload(ub)
@@ -849,10 +849,13 @@ def main():
         el_l = time.perf_counter() - t0
         gb = bo.download()
         e_l = float(np.max(np.abs(gb[:N] - o)) / np.max(np.abs(o)))
-        lines = {"vectors_per_step": B, "value": round(B * ls / el_l, 2), "unit": "transforms/s",
-                 "ms_per_step": round(1e3 * el_l / ls, 4), "max_rel_err_vs_oracle_first": e_l}
-        log(f"[bench] {B} vectors per step: {lines}")
-        del bi, bo
+        r = {"vectors_per_step": B, "value": round(B * ls / el_l, 2), "unit": "transforms/s",
+             "ms_per_step": round(1e3 * el_l / ls, 4), "max_rel_err_vs_oracle_first": e_l}
+        log(f"[bench] {B} vectors per step: {r}")
+        return r
+
+    lines = time_lines(args.lines) if args.lines > 1 else None
+    lines_more = time_lines(args.lines_more) if args.lines > 1 and args.lines_more > 1 else None
 
     # precision form 1 (split M2L matrices) on the same plan and input,
     # reported beside the FP64 value
@@ -1061,6 +1064,7 @@ def main():
         "split_form": split,
         "form2_split_products": form2,
         "lines_per_step": lines,
+        "lines_per_step_more": lines_more,
         "batched": batched,
         "dct2": dct,
         "field": field,
@@ -1076,8 +1080,9 @@ def main():
         bad = True
     if form2 is not None and form2.get("failed"):
         bad = True
-    if lines is not None and not lines["max_rel_err_vs_oracle_first"] <= tol:
-        bad = True
+    for ln in (lines, lines_more):
+        if ln is not None and not ln["max_rel_err_vs_oracle_first"] <= tol:
+            bad = True
     if batched is not None and not batched["max_rel_err_vs_oracle"] <= 1e-14 * math.log2(8192):
         bad = True
     if field is not None and not field["ok"]:
