@@ -428,6 +428,86 @@ __device__ __forceinline__ void split_unpack(const SplitView &sv, size_t block, 
   }
 }
 
+// One block per work-group (B < total_blocks(lu): multipoles summed from the
+// k_up contributions; or any block when only one fits the LDS), its lines
+// line0 .. line0 + nl - 1.  sm: m2l_wave_lds(M) + kM2lNT doubles.
+template <int MT>
+__device__ __forceinline__ void m2l_top(const Geo &g, const double *__restrict__ A,
+                                        double *__restrict__ wsall, size_t wsstride,
+                                        size_t line0, int nl, const SplitView &sv, double *sm,
+                                        int B) {
+  const int M = MT > 0 ? MT : g.M, M2 = 2 * M, MM = M * M, Mp = M + 1;
+  const int tid = threadIdx.x;
+  const WsLayout lay = ws_layout(g);
+  const int WL = m2l_wave_lds(M);
+  int lev = 0;
+  while ((int)total_blocks(lev + 1) <= B) lev++;
+  const int b = B - (int)total_blocks(lev);
+  double *Al = sm, *wl = sm + 3 * M * Mp, *red = sm + WL;
+  if (sv.rec) {
+    split_unpack<kM2lNT>(sv, (size_t)B, 3 * MM, Al, tid);
+  } else {
+    const double *Ab = A + (size_t)(3 * total_blocks(lev) + 3 * b) * MM;
+    double va[8];
+#pragma unroll
+    for (int q = 0; q < 8; q++) {
+      const int e = q * kM2lNT + tid;
+      va[q] = e < 3 * MM ? Ab[e] : 0.0;
+    }
+#pragma unroll
+    for (int q = 0; q < 8; q++) {
+      const int e = q * kM2lNT + tid;
+      if (e < 3 * MM) Al[(e / M) * Mp + e % M] = va[q];
+    }
+    for (int e = 8 * kM2lNT + tid; e < 3 * MM; e += kM2lNT) Al[(e / M) * Mp + e % M] = Ab[e];
+  }
+  const int nw = 2 * M2;  // multipole doubles of chunks 2b+2, 2b+3
+  for (int li = 0; li < nl; li++) {
+    double *ws = wsall + (line0 + li) * wsstride;
+    const double *w = ws + lay.w;
+    double *mo = ws + lay.m2l;
+    if (lev >= g.lu) {
+      const double *wb = w + (slot_base(lev) + 2 * (size_t)b + 2) * M2;
+      for (int o = tid; o < nw; o += kM2lNT) wl[o] = wb[o];
+    } else {
+      const size_t nd = (size_t)1 << (g.lu - lev);
+      const int ngrp = std::max(1, kM2lNT / nw);
+      for (int o0 = 0; o0 < nw; o0 += kM2lNT) {
+        const int o = o0 + tid % std::min(nw, kM2lNT), grp = tid / std::min(nw, kM2lNT);
+        double acc = 0;
+        if (grp < ngrp && o < nw) {
+          const int ci = o / M2, oo = o % M2;
+          const double *pp = ws + lay.part +
+                             ((size_t)lev * ((size_t)4 << g.lu) + (2 * (size_t)b + 2 + ci) * nd) * M2 + oo;
+          for (size_t d0 = grp; d0 < nd; d0 += 8 * (size_t)ngrp) {
+            double v[8];
+#pragma unroll
+            for (int q = 0; q < 8; q++) {
+              const size_t d = d0 + (size_t)q * ngrp;
+              v[q] = d < nd ? pp[d * M2] : 0.0;
+            }
+#pragma unroll
+            for (int q = 0; q < 8; q++) acc += v[q];
+          }
+        }
+        red[tid] = acc;
+        __syncthreads();
+        if (tid < std::min(nw, kM2lNT) && o0 + tid < nw) {
+          double sum = 0;
+          for (int gi = 0; gi < ngrp; gi++) sum += red[gi * std::min(nw, kM2lNT) + tid];
+          wl[o0 + tid] = sum;
+        }
+        __syncthreads();
+      }
+    }
+    __syncthreads();
+    m2l_block_compute<MT, true>(Al, wl, wl + nw, M, tid,
+                                mo + (slot_base(lev) + 2 * (size_t)b) * M2,
+                                mo + (slot_base(lev) + 2 * (size_t)b + 1) * M2);
+    __syncthreads();  // wl and the result scratch are reused by the next line
+  }
+}
+
 template <int MT>
 __global__ void __launch_bounds__(kM2lNT) k_m2l(Geo g, const double *__restrict__ A,
                                                 double *__restrict__ wsall, size_t wsstride,
@@ -447,74 +527,7 @@ __global__ void __launch_bounds__(kM2lNT) k_m2l(Geo g, const double *__restrict_
   if ((int)blockIdx.x < ntop || bpw == 1) {
     // ---- one block per work-group: every block above lu (multipoles summed
     // from the k_up contributions), and every block when bpw == 1
-    const int B = blockIdx.x;
-    if (B >= ntot) return;
-    int lev = 0;
-    while ((int)total_blocks(lev + 1) <= B) lev++;
-    const int b = B - (int)total_blocks(lev);
-    double *Al = sm, *wl = sm + 3 * M * Mp, *red = sm + WL;
-    if (sv.rec) {
-      split_unpack<kM2lNT>(sv, (size_t)B, 3 * MM, Al, tid);
-    } else {
-      const double *Ab = A + (size_t)(3 * total_blocks(lev) + 3 * b) * MM;
-      double va[8];
-#pragma unroll
-      for (int q = 0; q < 8; q++) {
-        const int e = q * kM2lNT + tid;
-        va[q] = e < 3 * MM ? Ab[e] : 0.0;
-      }
-#pragma unroll
-      for (int q = 0; q < 8; q++) {
-        const int e = q * kM2lNT + tid;
-        if (e < 3 * MM) Al[(e / M) * Mp + e % M] = va[q];
-      }
-      for (int e = 8 * kM2lNT + tid; e < 3 * MM; e += kM2lNT) Al[(e / M) * Mp + e % M] = Ab[e];
-    }
-    const int nw = 2 * M2;  // multipole doubles of chunks 2b+2, 2b+3
-    for (int li = 0; li < nl; li++) {
-      double *ws = wsall + (line0 + li) * wsstride;
-      const double *w = ws + lay.w;
-      double *mo = ws + lay.m2l;
-      if (lev >= g.lu) {
-        const double *wb = w + (slot_base(lev) + 2 * (size_t)b + 2) * M2;
-        for (int o = tid; o < nw; o += kM2lNT) wl[o] = wb[o];
-      } else {
-        const size_t nd = (size_t)1 << (g.lu - lev);
-        const int ngrp = std::max(1, kM2lNT / nw);
-        for (int o0 = 0; o0 < nw; o0 += kM2lNT) {
-          const int o = o0 + tid % std::min(nw, kM2lNT), grp = tid / std::min(nw, kM2lNT);
-          double acc = 0;
-          if (grp < ngrp && o < nw) {
-            const int ci = o / M2, oo = o % M2;
-            const double *pp = ws + lay.part +
-                               ((size_t)lev * ((size_t)4 << g.lu) + (2 * (size_t)b + 2 + ci) * nd) * M2 + oo;
-            for (size_t d0 = grp; d0 < nd; d0 += 8 * (size_t)ngrp) {
-              double v[8];
-#pragma unroll
-              for (int q = 0; q < 8; q++) {
-                const size_t d = d0 + (size_t)q * ngrp;
-                v[q] = d < nd ? pp[d * M2] : 0.0;
-              }
-#pragma unroll
-              for (int q = 0; q < 8; q++) acc += v[q];
-            }
-          }
-          red[tid] = acc;
-          __syncthreads();
-          if (tid < std::min(nw, kM2lNT) && o0 + tid < nw) {
-            double sum = 0;
-            for (int gi = 0; gi < ngrp; gi++) sum += red[gi * std::min(nw, kM2lNT) + tid];
-            wl[o0 + tid] = sum;
-          }
-          __syncthreads();
-        }
-      }
-      __syncthreads();
-      m2l_block_compute<MT, true>(Al, wl, wl + nw, M, tid,
-                                  mo + (slot_base(lev) + 2 * (size_t)b) * M2,
-                                  mo + (slot_base(lev) + 2 * (size_t)b + 1) * M2);
-      __syncthreads();  // wl and the result scratch are reused by the next line
-    }
+    if ((int)blockIdx.x < ntot) m2l_top<MT>(g, A, wsall, wsstride, line0, nl, sv, sm, blockIdx.x);
     return;
   }
 
@@ -564,6 +577,72 @@ __global__ void __launch_bounds__(kM2lNT) k_m2l(Geo g, const double *__restrict_
     double *mo = wsall + (line0 + li) * wsstride + oofs;
     m2l_block_compute<MT, false>(Al, wl, wl + 2 * M2, M, lane, mo, mo + M2);
     wave_sync();  // wl and the result scratch are reused by the next line
+  }
+}
+
+// M2L of the blocks at levels >= lu, one block per wave, from At (rows across
+// lanes): lane r < 3M holds row r of the block (r = sub M + row) in
+// registers after M coalesced loads, so no block is staged in LDS and the
+// work-group needs only the multipoles' 4M doubles per wave; the rows' sums
+// are combined across lanes (chunk 2b: rows of sub 0 plus sub 1; chunk
+// 2b+1: sub 2) in the order k_m2l uses, so the results are the same.  The
+// first ntop work-groups take the blocks above lu as k_m2l does (one per
+// work-group, from A).  Grid (ntop + (ntot - ntop + 3) / 4, line groups),
+// kM2lNT threads, 8 (m2l_wave_lds(M) + kM2lNT) bytes of LDS.
+template <int MT>
+__global__ void __launch_bounds__(kM2lNT) k_m2l_t(Geo g, const double *__restrict__ A,
+                                                  const double *__restrict__ At,
+                                                  double *__restrict__ wsall, size_t wsstride,
+                                                  size_t nlines, int lpb) {
+  static_assert(MT > 0 && 3 * MT <= 64, "rows of a block across one wave");
+  constexpr int M = MT, M2 = 2 * MT, MM = MT * MT;
+  extern __shared__ double sm[];
+  const int lane = threadIdx.x & 63;
+  const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  const int ntop = (int)total_blocks(g.lu), ntot = (int)total_blocks(g.L);
+  const size_t line0 = (size_t)blockIdx.y * lpb;
+  const int nl = (int)std::min<size_t>((size_t)lpb, nlines - line0);
+  if ((int)blockIdx.x < ntop) {
+    m2l_top<MT>(g, A, wsall, wsstride, line0, nl, SplitView{}, sm, blockIdx.x);
+    return;
+  }
+  const int B = ntop + ((int)blockIdx.x - ntop) * 4 + wave;
+  if (B >= ntot) return;  // no work-group barrier below this point
+  int lev = g.lu;
+  while ((int)total_blocks(lev + 1) <= B) lev++;
+  const int b = B - (int)total_blocks(lev);
+  const WsLayout lay = ws_layout(g);
+  const size_t wofs = lay.w + (slot_base(lev) + 2 * (size_t)b + 2) * M2;  // chunks 2b+2, 2b+3
+  const size_t oofs = lay.m2l + (slot_base(lev) + 2 * (size_t)b) * M2;
+  const bool rowl = lane < 3 * M;
+  const double *Ab = At + (size_t)B * 3 * MM;
+  double a[M];
+#pragma unroll
+  for (int q = 0; q < M; q++) a[q] = rowl ? Ab[q * 3 * M + lane] : 0.0;
+  double *wl = sm + wave * 2 * M2;
+  const int sub = lane / M;
+  const double *wv = wl + (sub == 0 ? 0 : M2);  // [chunk][parity][M]
+  for (int li = 0; li < nl; li++) {
+    const double *wb = wsall + (line0 + li) * wsstride + wofs;
+    wl[lane] = wb[lane];
+    if (lane + 64 < 2 * M2) wl[lane + 64] = wb[lane + 64];
+    wave_sync();
+    double o0 = 0, o1 = 0;
+#pragma unroll
+    for (int q = 0; q < M; q++) {
+      o0 += a[q] * wv[q];
+      o1 += a[q] * wv[M + q];
+    }
+    const double p0 = __shfl(o0, lane + M, 64), p1 = __shfl(o1, lane + M, 64);
+    double *mo = wsall + (line0 + li) * wsstride + oofs;
+    if (lane < M) {
+      mo[lane] = o0 + p0;
+      mo[M + lane] = o1 + p1;
+    } else if (lane >= 2 * M && rowl) {
+      mo[M2 + lane - 2 * M] = o0;
+      mo[M2 + M + lane - 2 * M] = o1;
+    }
+    wave_sync();  // wl is rewritten for the next line
   }
 }
 
@@ -3318,7 +3397,13 @@ static int launch_fmm_m(fmm_plan_s *p, const Geo &g, const double *d_in, double 
     const SplitView sv{p->precision == 1 ? sp.rec : nullptr, sp.pos, sp.n64, sp.n32, sp.n16,
                        sp.rec_bytes};
     const bool mma = MT == 18 && m2l_on_mfma(p, nlines);
-    if (!mma) {
+    if (!mma && MT == 18 && p->t.At[dir] && !sv.rec && bpw == 4) {
+      // the same grid; the blocks below lu from At, with the LDS of one
+      // staged block (the top blocks') per work-group instead of four
+      launch_k(p->prof, k_m2l_t<MT == 18 ? 18 : 1>, gm, kM2lNT,
+          8 * (size_t)(m2l_wave_lds(g.M) + kM2lNT), st, g, p->t.A[dir], p->t.At[dir], ws,
+          wsstride, nlines, lpb);
+    } else if (!mma) {
       launch_k(p->prof, k_m2l<MT>, gm, kM2lNT, m2l_shared_bytes(g), st, g, p->t.A[dir], ws,
           wsstride, bpw, nlines, lpb, sv);
     } else {

@@ -162,6 +162,10 @@ struct DevTables {
   double *an = nullptr;  // C2L diagonal scale
   double *dn = nullptr;  // C2L band factor
   double *A[2] = {nullptr, nullptr};  // Nc x M x M per direction
+  // the same blocks with each block's 3M rows across lanes (M <= 21): block
+  // B's element (sub, row, col) at B 3 M^2 + col 3M + sub M + row, so that
+  // lane sub M + row of a wave reads its whole row in M coalesced loads
+  double *At[2] = {nullptr, nullptr};
   double *T = nullptr;   // 2 x s x M  [parity][row][k]
   double *U = nullptr;   // 2 x M x M  upward (child q -> parent), row-major
   double *D = nullptr;   // 2 x M x M  downward (parent -> child p), row-major
@@ -296,6 +300,9 @@ struct fmm_plan_s {
   // line calls of such plans of at most this many elements (lines x N) take
   // the same direct sums, all lines in one launch; L2C_DIRECT_LINES
   int direct_line_elems = 16384;
+  // M2L of the blocks below the top levels from At (rows in registers, no
+  // LDS staging: k_m2l_t) for the VALU M2L; L2C_M2L_T=0: k_m2l from A
+  bool m2l_t = true;
   // single direct transforms of N > 3072 on k_direct_band (the band's
   // register rings over LDS windows) rather than k_direct_rows (three loads
   // per term); L2C_DIRECT_BAND=0 selects k_direct_rows

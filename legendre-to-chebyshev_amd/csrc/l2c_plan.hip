@@ -205,6 +205,16 @@ __global__ void k_lambda_minus(double *lm, const double *__restrict__ xj, int L,
 // 2 Lambda((y-x)/2) / (Lambda((x+y)/2) (x+y)(x+y+1)(x-y+1))
 // (src/C/leg2cheb.c:964-969).  DCT-II: (4/M^2) C F C^T, row 0 and column 0
 // halved (src/C/dct.c:582-598).
+// Block blockIdx.x of A (3 M x M, rows sub M + row) into At with its rows
+// across lanes: element (r, col) at col 3M + r.
+__global__ void k_rows_across_lanes(const double *__restrict__ A, double *__restrict__ At, int M) {
+  const size_t n = 3 * (size_t)M * M, base = blockIdx.x * n;
+  for (int e = threadIdx.x; e < (int)n; e += blockDim.x) {
+    const int r = e / M, col = e % M;
+    At[base + (size_t)col * 3 * M + r] = A[base + e];
+  }
+}
+
 __global__ void k_build_A(double *A0, double *A1, const double *__restrict__ lm,
                           const double *__restrict__ xj, const double *__restrict__ ctab,
                           int L, int s, int M, int lagrange) {
@@ -508,6 +518,8 @@ static void destroy_fmm(fmm_plan_s *p) {
   dfree(p->t.dn);
   dfree(p->t.A[0]);
   dfree(p->t.A[1]);
+  dfree(p->t.At[0]);
+  dfree(p->t.At[1]);
   for (int d = 0; d < 2; d++) {
     dfree(p->t.sp[d].rec);
     dfree(p->t.sp[d].pos);
@@ -606,6 +618,7 @@ fmm_plan_s *create_fmm_impl(size_t N, size_t maxs, size_t M, size_t direction, s
   if (const char *e = getenv("L2C_DIRECT_MAXN")) p->direct_max_n = std::max(0, atoi(e));
   if (const char *e = getenv("L2C_DIRECT_BAND")) p->direct_band = atoi(e) != 0;
   if (const char *e = getenv("L2C_DIRECT_LINES")) p->direct_line_elems = std::max(0, atoi(e));
+  if (const char *e = getenv("L2C_M2L_T")) p->m2l_t = atoi(e) != 0;
   (void)hipGetDevice(&p->device);
   if (hipStreamCreateWithFlags(&p->stream, hipStreamNonBlocking) != hipSuccess) {
     set_error("hipStreamCreate failed");
@@ -836,6 +849,18 @@ fmm_plan_s *create_fmm_impl(size_t N, size_t maxs, size_t M, size_t direction, s
   k_build_A<<<(unsigned)Nc, 256, shm, st>>>(p->t.A[0], p->t.A[1], d_lm, d_xj, d_ctab, L, (int)s,
                                             Mi, (int)lagrange);
   L2C_HIP(hipGetLastError(), (destroy_fmm(p), nullptr));
+  if (p->m2l_t && M <= 21) {
+    // the row-across-lanes copy for k_m2l_t (Nc / 3 blocks of 3 M^2)
+    for (int d = 0; d < 2; d++) {
+      if (!p->t.A[d]) continue;
+      if (dmalloc(&p->t.At[d], Nc * MM, acct)) {
+        destroy_fmm(p);
+        return nullptr;
+      }
+      k_rows_across_lanes<<<(unsigned)(Nc / 3), 256, 0, st>>>(p->t.A[d], p->t.At[d], Mi);
+      L2C_HIP(hipGetLastError(), (destroy_fmm(p), nullptr));
+    }
+  }
 
   // multipole store and staging
   const size_t slots = slot_base(L);

@@ -92,3 +92,30 @@ def test_m2l_mma_fp64_lines(amd, monkeypatch, N, lines):
             outs.append(dout.download().reshape(lines, N))
         worst = max(relerr(outs[1][k], outs[0][k]) for k in range(lines))
         assert worst <= 1e-15, (d, worst)
+
+
+@pytest.mark.parametrize("N,lines", [(1 << 20, 1), (1 << 16, 1), (100000, 1), (1 << 18, 8),
+                                     (1 << 16, 16), (8191, 3)])
+def test_m2l_rows_across_lanes(amd, monkeypatch, N, lines):
+    """k_m2l_t (the M2L blocks below the top levels with their rows in
+    registers, read from the rows-across-lanes copy of the M2L matrices; the
+    default) against k_m2l (blocks staged in LDS, L2C_M2L_T=0): the same
+    sums in the same order, so bit-identical outputs."""
+    announce(f"m2l rows across lanes N={N} lines={lines}")
+    monkeypatch.setenv("L2C_M2L_T", "0")
+    p0 = amd.FMMPlan(N, 64, 18, amd.BOTH)
+    monkeypatch.delenv("L2C_M2L_T")
+    p1 = amd.FMMPlan(N, 64, 18, amd.BOTH)
+    x = np.random.default_rng(N + 7).uniform(-1, 1, lines * N)
+    din, dout = amd.DeviceBuffer(x.size), amd.DeviceBuffer(x.size)
+    din.upload(x)
+    for d in (0, 1):
+        outs = []
+        for p in (p0, p1):
+            if lines == 1:
+                p.execute_device(din.ptr, dout.ptr, d)
+            else:
+                p.execute_lines_device(din.ptr, dout.ptr, d, lines, lines, N, 0, 1)
+            p.synchronize()
+            outs.append(dout.download())
+        assert np.array_equal(outs[0], outs[1]), d
