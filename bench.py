@@ -117,9 +117,10 @@ def _counter_means(csv_dir, counter, copy_kib):
                     name = name[5:]
                 if name.startswith("l2c::"):
                     name = name[5:]
-                # profile slots: k_down covers the downward kernels (k_down8r,
-                # k_down8, k_down)
-                for tag, slot in (("k_up", "k_up"), ("k_m2l", "k_m2l"), ("k_down8r", "k_down"),
+                # profile slots: k_m2l covers k_m2l_t and k_m2l, k_down the
+                # downward kernels (k_down8r, k_down8, k_down)
+                for tag, slot in (("k_up", "k_up"), ("k_m2l_t", "k_m2l"), ("k_m2l", "k_m2l"),
+                                  ("k_down8r", "k_down"),
                                   ("k_down8", "k_down"), ("k_down", "k_down"),
                                   ("k_direct", "k_direct")):
                     if name.startswith(tag + "<") or name.startswith(tag + "(") or name == tag:
